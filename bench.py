@@ -1,0 +1,215 @@
+"""Benchmark of the direct-sum softened gravity step on one MI355X.
+
+A step is one full gravity evaluation of the giant-star workload with all
+inputs already resident in HBM: validation, Morton sort, gather, tile boxes,
+far-field pair kernel, near-field kernel and the FP64 reduction, i.e. every
+pair (i, j) of N = 1,048,576 particles (N^2 ordered pair-interactions,
+including i == j).  The FP32+compensated kernel is the product; the plain FP64
+kernel is timed the same way beside it.
+
+Prints ONE JSON line (rank 0, N = 1 GPU only).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("G pair-interactions/s at N=1M: FP32+comp vs FP64 GPU, max rel acc err ≤1e-5")
+FLOPS_PER_PAIR = 19           # 3 dr + 5 r^2 + 1 rsqrt + 3 (r^-2, m r^-1, m r^-3) + 6 acc + 1 pot
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector (AMD spec)
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    from starsmasher_amd import ics
+    if name == "giant1m":
+        d = ics.giant_with_core(1 << 20)
+        desc = ("giant star N=1048576: n=3 polytrope core (60%) + r^-2.5 envelope to r=300, "
+                "masses over 4 decades, hp over 3 decades, one point mass of 30% total mass "
+                "(hp=0.1, range2=-0.04); cubic spline (nkernel=0)")
+    elif name == "twopoly262k":
+        d = ics.two_polytropes(1 << 18)
+        desc = "two n=1.5 polytropes N=262144 (mass ratio 0.6, radii 1.0/0.8, 3.0 apart); nkernel=0"
+    elif name == "plummer16k":
+        d = ics.plummer(16384)
+        desc = "Plummer sphere N=16384 (a=1, r<10a, m=1/N); nkernel=0"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return d, desc
+
+
+def time_precision(g, dev, n, prec, steps, warmup):
+    import torch
+    out = [torch.empty(n, dtype=torch.float64, device="cuda:0") for _ in range(4)]
+    for _ in range(warmup):
+        g.compute_device(n, *dev, *out, nkernel=0, precision=prec)
+        g.synchronize()
+    torch.cuda.synchronize()
+    far_ms, near_ms, prep_ms = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.compute_device(n, *dev, *out, nkernel=0, precision=prec)
+        g.synchronize()
+        p, f, nr = g.last_timings()
+        prep_ms.append(p)
+        far_ms.append(f)
+        near_ms.append(nr)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    visits, nearv = g.last_tile_stats()
+    res = [o.cpu().numpy() for o in out]
+    return {
+        "ms_per_step": (t1 - t0) * 1e3 / steps,
+        "far_ms": float(np.mean(far_ms)), "near_ms": float(np.mean(near_ms)),
+        "prep_ms": float(np.mean(prep_ms)), "visits": visits, "near_visits": nearv,
+        "result": res,
+    }
+
+
+def rel_err(a, b):
+    ea = np.linalg.norm(np.stack(a[:3], -1) - np.stack(b[:3], -1), axis=1) / \
+        np.linalg.norm(np.stack(b[:3], -1), axis=1)
+    ep = np.abs(a[3] - b[3]) / np.abs(b[3])
+    return ea, ep
+
+
+def cpu_baseline(d, seconds, nthreads):
+    """The oracle's C restatement of the reference's symmetric CPU gravity
+    loop (cpu_grav.f:1-381) on a bounded row sample of the same workload."""
+    import oracle
+    args = (d["x"], d["y"], d["z"], d["m"], d["hp"], d["u"])
+    n = d["x"].size
+    # calibrate on 4 rows per thread
+    rows = 4 * nthreads
+    t = time.perf_counter()
+    ev, *_ = oracle.cpu_twin_rows_parallel(*args, 0, rows - 1, 0, nthreads)
+    dt = time.perf_counter() - t
+    rows = int(max(rows, min(n - 1, rows * seconds / max(dt, 1e-3))))
+    t = time.perf_counter()
+    ev, *_ = oracle.cpu_twin_rows_parallel(*args, 0, rows - 1, 0, nthreads)
+    dt = time.perf_counter() - t
+    # each symmetric evaluation covers two ordered pair-interactions (i<-j, j<-i)
+    return {"value": 2.0 * ev / dt / 1e9, "unit": "Gpairs/s", "cores": nthreads, "kind": "port",
+            "sample": f"rows 1..{rows} of the symmetric j>=i loop over all {n} particles "
+                      f"({ev} evaluations, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="giant1m")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the gravity step shards by i-range with replicated j; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from starsmasher_amd import SG_PREC_FP32C, SG_PREC_FP64, Gravity
+
+    log(f"bench: generating workload {args.workload}")
+    d, desc = make_workload(args.workload)
+    n = d["x"].size
+    dev = [torch.from_numpy(np.ascontiguousarray(d[k])).to("cuda:0")
+           for k in ("x", "y", "z", "m", "range2")]
+    g = Gravity(0)
+    wg, ipl, js = g.get_config(SG_PREC_FP32C)
+
+    log(f"bench: FP32C n={n} steps={args.steps} warmup={args.warmup}")
+    r32 = time_precision(g, dev, n, SG_PREC_FP32C, args.steps, args.warmup)
+    log(f"bench: FP32C {r32['ms_per_step']:.2f} ms/step (far {r32['far_ms']:.2f} ms)")
+    r64 = None
+    if not args.no_fp64:
+        log("bench: FP64")
+        r64 = time_precision(g, dev, n, SG_PREC_FP64, max(1, args.steps // 2), 1)
+        log(f"bench: FP64 {r64['ms_per_step']:.2f} ms/step (far {r64['far_ms']:.2f} ms)")
+
+    pairs = float(n) * float(n)
+    value = pairs / (r32["ms_per_step"] * 1e-3) / 1e9
+    near_frac = r32["near_visits"] / max(1, r32["visits"])
+    padded = -(-n // 256) * 256
+    far_pairs = float(n) * padded * (1.0 - near_frac)
+    achieved = FLOPS_PER_PAIR * far_pairs / (r32["far_ms"] * 1e-3) / 1e12
+
+    acc = {}
+    if r64 is not None:
+        ea, ep = rel_err(r32["result"], r64["result"])
+        acc["fp32c_vs_fp64_kernel_all"] = {
+            "acc_max": float(ea.max()), "acc_median": float(np.median(ea)),
+            "pot_max": float(ep.max()), "pot_median": float(np.median(ep))}
+    if not args.no_oracle:
+        import oracle
+        rng = np.random.default_rng(0)
+        rows = rng.choice(n, 192, replace=False)
+        if r64 is not None:
+            rows = np.unique(np.concatenate([rows, np.argsort(ea)[-32:], np.argsort(ep)[-16:]]))
+        log(f"bench: oracle on {rows.size} rows")
+        ref = oracle.direct(d["x"], d["y"], d["z"], d["m"], d["range2"], nkernel=0, rows=rows,
+                            nthreads=min(16, os.cpu_count() or 1))
+        for tag, rr in (("fp32c", r32), ("fp64", r64)):
+            if rr is None:
+                continue
+            ea_o, ep_o = rel_err([x[rows] for x in rr["result"]], ref)
+            acc[f"{tag}_vs_oracle_rows"] = {
+                "rows": int(rows.size), "acc_max": float(ea_o.max()),
+                "acc_median": float(np.median(ea_o)), "pot_max": float(ep_o.max()),
+                "pot_median": float(np.median(ep_o))}
+
+    cpu = None
+    if not args.no_cpu:
+        nth = min(16, os.cpu_count() or 1)
+        log(f"bench: CPU baseline on {nth} threads")
+        cpu = cpu_baseline(d, args.cpu_seconds, nth)
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gpairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r32["ms_per_step"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": desc, "n": n, "nkernel": 0, "tile": {"wg": wg, "ipl": ipl,
+                   "jsplit": js}},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": None, "kernel": "k_far_fp32c",
+                     "flops_per_pair": FLOPS_PER_PAIR,
+                     "far_pairs_per_launch": far_pairs, "far_kernel_ms": round(r32["far_ms"], 3)},
+        "phases_ms": {"prep": round(r32["prep_ms"], 3), "far": round(r32["far_ms"], 3),
+                      "near_and_reduce": round(r32["near_ms"], 3)},
+        "near_tile_fraction": round(near_frac, 5),
+        "accuracy": acc,
+        "cpu_baseline": cpu,
+    }
+    if r64 is not None:
+        v64 = pairs / (r64["ms_per_step"] * 1e-3) / 1e9
+        a64 = FLOPS_PER_PAIR * far_pairs / (r64["far_ms"] * 1e-3) / 1e12
+        line["fp64_kernel"] = {"value": round(v64, 3), "unit": "Gpairs/s",
+                               "ms_per_step": round(r64["ms_per_step"], 3),
+                               "far_kernel_ms": round(r64["far_ms"], 3),
+                               "far_tflops": round(a64, 3),
+                               "frac_of_fp64_peak": round(a64 / PEAK_FP64_TFLOPS, 4)}
+        line["fp32c_over_fp64"] = round(value / v64, 3)
+    g.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,161 @@
+/*
+ * sphgrav.h -- C ABI of the MI355X (gfx950) direct-summation softened
+ * self-gravity library (starsmasher_amd/libsphgrav_hip.so).
+ *
+ * Two layers:
+ *
+ * 1. The three Fortran-callable symbols of the reference gravity library,
+ *    exported verbatim so an unmodified Fortran host links against this
+ *    library instead of libSPHgrav.a.  All arguments are by reference
+ *    (gfortran implicit interface, default integer = int32).
+ *
+ * 2. An sg_* API with int status returns, host-pointer and device-pointer
+ *    entry points, used by the legacy layer, the tests, bench.py and new
+ *    Fortran call sites (starsmasher_amd/fortran/sphgrav_mod.f90).
+ *
+ * Physics (units G = 1): for every i in the i-range and every j in 0..n-1,
+ *   a_i   += m_j * 0.5*(F(r,H_i) + F(r,H_j)) * (x_j - x_i)
+ *   phi_i += m_j * 0.5*(P(r,H_i) + P(r,H_j))
+ * with H = sqrt|range2| (= 2 hp), F = A(r/H)/H^3 and P = P(r/H)/H when r < H
+ * and F = r^-3, P = -1/r otherwise (cubic spline or Wendland W3,3 / C4 by
+ * nkernel).  The i == j self pair adds -c*flag_j*(m_j/H_i + m_j/H_j) to phi
+ * (c = 1.4, 1.9140625, 1.71875), flag_j = range2_j > 0 (gas) and 0 for point
+ * particles (range2 < 0).  A distinct pair at exactly r = 0 contributes the
+ * finite r->0 limit -c*(m_j/H_i + m_j/H_j) (the reference CUDA kernel
+ * produces NaN there, grav_force_direct.cu:110-112).
+ */
+#ifndef SPHGRAV_H
+#define SPHGRAV_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Layer 1: reference ABI (pbe/src/SPHgrav_lib2/grav_force_direct.cu:648-677) */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces firsthalf_grav_forces_ (grav_force_direct.cu:650-656).
+ * j-set = particles 1..*n, i-set = *n_lower .. *n_lower + *my_length - 1
+ * (1-based).  Copies the host arrays synchronously (the caller may reuse
+ * them on return), launches the computation asynchronously and returns.
+ * *q is ignored (legacy threads-per-particle knob, as in the reference).
+ * On a bad argument or HIP error: prints a message and exits with
+ * EXIT_FAILURE (the reference asserts / CUDA_SAFE_CALL exits), unless the
+ * environment sets SPHGRAV_NOABORT=1, in which case the call records the
+ * status (sg_last_status) and returns. */
+void firsthalf_grav_forces_(int *n, int *n_lower, int *my_length,
+                            double *px, double *py, double *pz,
+                            double *mass, double *range2, int *q, int *nkernel);
+
+/* Replaces lasthalf_grav_forces_ (grav_force_direct.cu:658-664).
+ * Blocks until the preceding firsthalf's results are ready and writes
+ * ax/ay/az/pot[n_lower-1 .. n_lower+my_length-2] (0-based); other entries
+ * are untouched.  *n and *myrank are ignored, as in the reference.  Prints
+ * " >>> SPHgrav_lib took %g sec" to stderr unless SPHGRAV_QUIET=1. */
+void lasthalf_grav_forces_(int *n, double *ax, double *ay, double *az,
+                           double *pot, int *myrank);
+
+/* Replaces gpu_init_dev_ (grav_force_direct.cu:672-675): select the HIP
+ * device for this process.  *theta is ignored (tree-library opening angle). */
+void gpu_init_dev_(int *device, double *theta);
+
+/* ------------------------------------------------------------------------ */
+/* Layer 2: sg_* API                                                         */
+/* ------------------------------------------------------------------------ */
+
+enum sg_status {
+    SG_OK = 0,
+    SG_ERR_ARG = 1,          /* bad size / range / null pointer            */
+    SG_ERR_NKERNEL = 2,      /* nkernel not in {0,1,2}                      */
+    SG_ERR_MASS = 3,         /* mass <= 0 (grav_force_direct.cu:602)        */
+    SG_ERR_RANGE2 = 4,       /* range2 == 0 (grav_force_direct.cu:601)      */
+    SG_ERR_NONFINITE = 5,    /* NaN / Inf in positions, mass or range2      */
+    SG_ERR_HIP = 6,          /* HIP runtime error                           */
+    SG_ERR_NO_DEVICE = 7,    /* no HIP device / device index out of range   */
+    SG_ERR_STATE = 8,        /* lasthalf without a pending firsthalf         */
+    SG_ERR_CONFIG = 9        /* unsupported tile configuration              */
+};
+
+enum sg_precision {
+    SG_PREC_FP32C = 0,       /* FP32 pair arithmetic, hi/lo positions relative
+                                to an FP64 tile origin, FP64 cross-tile sums   */
+    SG_PREC_FP64 = 1         /* plain FP64 pair arithmetic                     */
+};
+
+const char *sg_status_string(int status);
+int sg_last_status(void);         /* status of the last legacy-layer call */
+int sg_device_count(int *count);
+
+typedef struct sg_context sg_context;
+
+/* Create a context bound to HIP device `device` (owns device buffers that
+ * grow on demand and a default-priority stream). */
+int sg_create(int device, sg_context **out);
+int sg_destroy(sg_context *ctx);
+
+/* Enqueue the context's work on `stream` (a hipStream_t of the context's
+ * device, e.g. the caller's current torch stream) instead of its own stream;
+ * NULL restores the context's own stream.  Waits for work already queued. */
+int sg_set_stream(sg_context *ctx, void *stream);
+
+/* Device-resident evaluation.  All array pointers are DEVICE pointers on the
+ * context's device.  Inputs: n particles (x,y,z,m,range2 as in layer 1).
+ * i-range: 0-based [ibeg, ibeg+ni).  Outputs ax..pot have length ni and
+ * receive the i-range in input order.  Work is enqueued on the context's
+ * stream; call sg_synchronize before reading results.  Input validation runs
+ * on the device; its verdict is returned by sg_synchronize. */
+int sg_compute_device(sg_context *ctx, int n, int ibeg, int ni,
+                      const double *x, const double *y, const double *z,
+                      const double *m, const double *range2,
+                      int nkernel, int precision,
+                      double *ax, double *ay, double *az, double *pot);
+
+/* Wait for the context's stream; returns the validation status of the last
+ * sg_compute_device (SG_ERR_MASS, SG_ERR_RANGE2, SG_ERR_NONFINITE) or SG_OK. */
+int sg_synchronize(sg_context *ctx);
+
+/* Milliseconds of the main pair kernel(s) of the last completed evaluation,
+ * measured with HIP events on the context's stream (call after
+ * sg_synchronize). */
+int sg_last_kernel_ms(sg_context *ctx, float *ms);
+
+/* Host-pointer evaluation (blocking): copies in, computes, copies out.
+ * Same semantics as sg_compute_device with host arrays. */
+int sg_compute_host(sg_context *ctx, int n, int ibeg, int ni,
+                    const double *x, const double *y, const double *z,
+                    const double *m, const double *range2,
+                    int nkernel, int precision,
+                    double *ax, double *ay, double *az, double *pot);
+
+/* Tile configuration of the pair kernel for a precision.
+ * wg: threads per workgroup (64, 128 or 256); ipl: i-particles per lane
+ * (1, 2 or 4); jsplit: number of j-partitions per i-block (>= 1, partial sums
+ * are reduced in FP64 in a fixed order); 0 for jsplit selects it from n. */
+int sg_set_config(sg_context *ctx, int precision, int wg, int ipl, int jsplit);
+int sg_get_config(sg_context *ctx, int precision, int *wg, int *ipl, int *jsplit);
+
+/* HIP-event timings of the last completed evaluation, in milliseconds:
+ * prep = validation, curve sort, gather and tile boxes; far = the far-field
+ * pair kernel alone; near = near-field kernel plus the final reduction.
+ * Any pointer may be NULL. */
+int sg_last_timings(sg_context *ctx, float *prep_ms, float *far_ms, float *near_ms);
+
+/* Statistics of the last evaluation: number of (wave, j-tile) visits and
+ * how many took the softened (near-field) path. */
+int sg_last_tile_stats(sg_context *ctx, long long *visits, long long *near_visits);
+
+/* Legacy-layer equivalents with status returns (used by layer 1). */
+int sg_init_device(int device);
+int sg_firsthalf(int n, int n_lower, int my_length,
+                 const double *x, const double *y, const double *z,
+                 const double *m, const double *range2, int nkernel);
+int sg_lasthalf(double *ax, double *ay, double *az, double *pot);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPHGRAV_H */

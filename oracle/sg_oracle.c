@@ -1,0 +1,647 @@
+/*
+ * sg_oracle.c -- FP64 CPU restatement of StarSmasher's direct-summation
+ * softened self-gravity.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * starsmasher_amd/csrc; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it.  Nothing in the product links it.
+ *
+ * Reference files restated here (paths relative to the reference checkout,
+ * "pbe" = parallel_bleeding_edge):
+ *   pbe/src/SPHgrav_lib2/grav_force_direct.cu:55-396   pair functions (nkernel 0,1,2)
+ *   pbe/src/SPHgrav_lib2/grav_force_direct.cu:398-502  full i x j loop (non-symmetric)
+ *   pbe/src/SPHgrav_lib2/grav_force_direct.cu:599-609  range2 decode: H=sqrt|range2|, flag=range2>0
+ *   pbe/src/cpu_grav.f:1-381                           symmetric j>=i CPU twin
+ *   pbe/src/main.f:297-349                             interaction-balanced i split
+ *   pbe/src/kernels.f:215-279                          dphi/dh closed forms (adjoint check)
+ *
+ * Pinning (what checks each function; see tests/test_oracle.py):
+ *   sgo_direct        golden grpot of the reference's own start dumps (cubic),
+ *                     brute force of 2-3 particle cases against scipy quad-free
+ *                     closed forms, -grad(phi) == a by central differences,
+ *                     momentum conservation, Newtonian limit, q=1 continuity.
+ *   sgo_cpu_twin      the reference's own cpu_grav.f compiled from its source
+ *                     (oracle/_ref), bit-for-bit modulo summation order.
+ *   sgo_vjp           central finite differences of sgo_loss_particle and
+ *                     kernels.f dphidh (cubic) / dphi332hdh / dphic42hdh.
+ *   sgo_partition     hand-worked splits and the invariants of main.f:297-349.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Kernel polynomials.  A(q) is the force factor (a = m A / H^3 * dr),       */
+/* P(q) the potential factor (phi = m P / H), q = r / H, H = 2 hp.           */
+/* Decimal literals follow cpu_grav.f so FP64 results agree with it.         */
+/* ------------------------------------------------------------------------ */
+
+/* cubic spline: cpu_grav.f:91-117, grav_force_direct.cu:86-101 */
+static void cubic_AP(double q, double invq, double *A, double *P)
+{
+    const double q2 = q * q, q3 = q * q2;
+    const double invq3 = invq * invq * invq;
+    if (q < 0.5) {
+        *A = 10.666666666667 + q2 * (32.0 * q - 38.4);
+        *P = -2.8 + q2 * (5.333333333333 + q2 * (6.4 * q - 9.6));
+    } else {
+        *A = 21.333333333333 - 48.0 * q + 38.4 * q2 - 10.666666666667 * q3
+             - 0.066666666667 * invq3;
+        *P = -3.2 + 0.066666666667 * invq
+             + q2 * (10.666666666667 + q * (-16.0 + q * (9.6 - 2.133333333333 * q)));
+    }
+}
+
+/* Wendland W3,3: cpu_grav.f:210-243, grav_force_direct.cu:211-221 */
+static void w33_AP(double q, double *A, double *P)
+{
+    const double q2 = q * q;
+    *A = 28.4375 + q2 * (-187.6875 + q2 * (804.375 + q2 * (-4379.375 + q * (9009.0
+         + q * (-8957.8125 + q * (5005.0 + q * (-1515.9375 + 195.0 * q)))))));
+    *P = -3.828125 + q2 * (14.21875 + q2 * (-46.921875 + q2 * (134.0625 + q2 * (
+         -547.421875 + q * (1001.0 + q * (-895.78125 + q * (455.0 + q * (
+         -126.328125 + 15.0 * q))))))));
+}
+
+/* Wendland C4: cpu_grav.f:321-350, grav_force_direct.cu:317-327 */
+static void c4_AP(double q, double *A, double *P)
+{
+    const double q2 = q * q;
+    *A = 20.625 + q2 * (-115.5 + q2 * (618.75 + q * (-1155.0 + q * (962.5
+         + q * (-396.0 + 65.625 * q)))));
+    *P = -3.4375 + q2 * (10.3125 + q2 * (-28.875 + q2 * (103.125 + q * (-165.0
+         + q * (120.3125 + q * (-44.0 + 6.5625 * q))))));
+}
+
+/* Self-potential coefficient c = -P(0)/2: grav_force_direct.cu:112,232,336 */
+static double self_coeff(int nkernel)
+{
+    return nkernel == 0 ? 1.4 : (nkernel == 1 ? 1.9140625 : 1.71875);
+}
+
+static void kernel_AP(int nkernel, double q, double invq, double *A, double *P)
+{
+    if (nkernel == 0) cubic_AP(q, invq, A, P);
+    else if (nkernel == 1) w33_AP(q, A, P);
+    else c4_AP(q, A, P);
+}
+
+/* Derivatives dA/dq and dP/dq, used only by the adjoint.  The polynomial
+ * forms above are differentiated term by term. */
+static void kernel_dAP(int nkernel, double q, double *dA, double *dP)
+{
+    const double q2 = q * q;
+    if (nkernel == 0) {
+        if (q < 0.5) {
+            *dA = q * (96.0 * q - 76.8);
+            *dP = q * (10.666666666666 + q2 * (32.0 * q - 38.4));
+        } else {
+            const double invq = 1.0 / q;
+            *dA = -48.0 + 76.8 * q - 32.000000000001 * q2
+                  + 0.200000000001 * invq * invq * invq * invq;
+            *dP = -0.066666666667 * invq * invq
+                  + q * (21.333333333334 + q * (-48.0 + q * (38.4 - 10.666666666665 * q)));
+        }
+    } else if (nkernel == 1) {
+        /* d/dq of w33_AP */
+        *dA = q * (2.0 * -187.6875 + q2 * (4.0 * 804.375 + q2 * (6.0 * -4379.375
+              + q * (7.0 * 9009.0 + q * (8.0 * -8957.8125 + q * (9.0 * 5005.0
+              + q * (10.0 * -1515.9375 + 11.0 * 195.0 * q)))))));
+        *dP = q * (2.0 * 14.21875 + q2 * (4.0 * -46.921875 + q2 * (6.0 * 134.0625
+              + q2 * (8.0 * -547.421875 + q * (9.0 * 1001.0 + q * (10.0 * -895.78125
+              + q * (11.0 * 455.0 + q * (12.0 * -126.328125 + 13.0 * 15.0 * q))))))));
+    } else {
+        *dA = q * (2.0 * -115.5 + q2 * (4.0 * 618.75 + q * (5.0 * -1155.0
+              + q * (6.0 * 962.5 + q * (7.0 * -396.0 + 8.0 * 65.625 * q)))));
+        *dP = q * (2.0 * 10.3125 + q2 * (4.0 * -28.875 + q2 * (6.0 * 103.125
+              + q * (7.0 * -165.0 + q * (8.0 * 120.3125 + q * (9.0 * -44.0
+              + 10.0 * 6.5625 * q))))));
+    }
+}
+
+/* Exported thin wrappers so tests can pin the polynomials themselves
+ * (continuity at q=1, P'(q) = q A(q), dphidh identities). */
+void sgo_kernel_AP(int nkernel, double q, double *A, double *P)
+{
+    kernel_AP(nkernel, q, q > 0 ? 1.0 / q : 0.0, A, P);
+}
+
+void sgo_kernel_dAP(int nkernel, double q, double *dA, double *dP)
+{
+    kernel_dAP(nkernel, q, dA, dP);
+}
+
+/* ------------------------------------------------------------------------ */
+/* One ordered pair (i <- j) with the library's non-symmetric semantics      */
+/* (grav_force_direct.cu:55-396).  Inputs are FP64; H = support radius.      */
+/*                                                                            */
+/* self_rule for r == 0:                                                      */
+/*   0 = "limit": i == j gives -c*flag_j*(m_j/H_i + m_j/H_j); a distinct      */
+/*       coincident pair gives the finite r->0 limit -c*(m_j/H_i + m_j/H_j).  */
+/*       This is the HIP library's documented behaviour.                      */
+/*   1 = "gpu": the reference kernel's rule (:110): self term iff H and m are */
+/*       equal, otherwise the (g-1)*m/r = 0*inf NaN it produces.              */
+/* ------------------------------------------------------------------------ */
+typedef struct { double ax, ay, az, pot; } pairsum;
+
+static void pair_direct(int nkernel, int self_rule, int same_index,
+                        double dx, double dy, double dz,
+                        double Hi, double Hj, double mi, double mj, double flagj,
+                        pairsum *s)
+{
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double hi2 = Hi * Hi, hj2 = Hj * Hj;
+    const double rinv = 1.0 / sqrt(r2);              /* inf at r2 == 0 */
+    const double mrinv1 = rinv * mj;
+    const double mrinv3 = rinv * rinv * mrinv1;
+
+    if (r2 >= fmax(hi2, hj2)) {                      /* :69-75 Newtonian branch */
+        s->ax += mrinv3 * dx; s->ay += mrinv3 * dy; s->az += mrinv3 * dz;
+        s->pot -= mrinv1;
+        return;
+    }
+    if (r2 == 0.0) {                                 /* only reachable when r2 < min(h2) */
+        const double c = self_coeff(nkernel);
+        const double base = -c * (mj / Hi + mj / Hj);
+        if (self_rule == 0) {
+            s->pot += same_index ? flagj * base : base;
+        } else {
+            if (hi2 == hj2 && mi == mj) s->pot += flagj * base;
+            else s->pot += NAN;                      /* reference kernel: 0*inf */
+        }
+        return;
+    }
+    /* softened: average of two one-sided terms, each softened iff r2 <= H_k^2
+     * (the g flags of :106, equivalent to the three-way branch :76-180). */
+    const double r = sqrt(r2);
+    double gacc = 0.0, gpot = 0.0;
+    const double Hs[2] = {Hi, Hj};
+    const double h2s[2] = {hi2, hj2};
+    for (int k = 0; k < 2; ++k) {
+        if (r2 <= h2s[k]) {
+            const double H = Hs[k];
+            const double invq = rinv * H, q = 1.0 / invq;
+            double A, P;
+            kernel_AP(nkernel, q, invq, &A, &P);
+            gacc += mj / (H * H * H) * A;
+            gpot += mj / H * P;
+        } else {
+            gacc += mrinv3;
+            gpot -= mrinv1;
+        }
+    }
+    (void)r;
+    gacc *= 0.5; gpot *= 0.5;
+    s->ax += gacc * dx; s->ay += gacc * dy; s->az += gacc * dz;
+    s->pot += gpot;
+}
+
+static inline double decode_H(double range2) { return sqrt(range2 > 0.0 ? range2 : -range2); }
+static inline double decode_flag(double range2) { return range2 > 0.0 ? 1.0 : 0.0; }
+
+static int check_kernel(int nkernel) { return nkernel >= 0 && nkernel <= 2; }
+
+/*
+ * Full i x j direct sum (grav_force_direct.cu:398-502 semantics, FP64).
+ * i runs over rows[0..nrows) (0-based particle indices), j over 0..n-1.
+ * Outputs are indexed by row position (length nrows).
+ * Returns 0 on success, -1 on a bad argument.
+ */
+int sgo_direct_rows(int n, const int64_t *rows, int64_t nrows,
+                    const double *x, const double *y, const double *z,
+                    const double *m, const double *range2,
+                    int nkernel, int self_rule,
+                    double *ax, double *ay, double *az, double *pot, int nthreads)
+{
+    if (n < 0 || nrows < 0 || !check_kernel(nkernel)) return -1;
+    for (int64_t r = 0; r < nrows; ++r)
+        if (rows[r] < 0 || rows[r] >= n) return -1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t r = 0; r < nrows; ++r) {
+        const int64_t i = rows[r];
+        const double xi = x[i], yi = y[i], zi = z[i];
+        const double Hi = decode_H(range2[i]), mi = m[i];
+        pairsum s = {0, 0, 0, 0};
+        for (int64_t j = 0; j < n; ++j) {
+            pair_direct(nkernel, self_rule, i == j, x[j] - xi, y[j] - yi, z[j] - zi,
+                        Hi, decode_H(range2[j]), mi, m[j], decode_flag(range2[j]), &s);
+        }
+        ax[r] = s.ax; ay[r] = s.ay; az[r] = s.az; pot[r] = s.pot;
+    }
+    return 0;
+}
+
+/* Contiguous i-range form mirroring firsthalf_grav_forces_ (0-based ibeg). */
+int sgo_direct(int n, int ibeg, int ni,
+               const double *x, const double *y, const double *z,
+               const double *m, const double *range2, int nkernel, int self_rule,
+               double *ax, double *ay, double *az, double *pot, int nthreads)
+{
+    if (ibeg < 0 || ni < 0 || ibeg + ni > n) return -1;
+    int64_t *rows = (int64_t *)malloc(sizeof(int64_t) * (size_t)(ni > 0 ? ni : 1));
+    if (!rows) return -1;
+    for (int k = 0; k < ni; ++k) rows[k] = ibeg + k;
+    const int rc = sgo_direct_rows(n, rows, ni, x, y, z, m, range2, nkernel, self_rule,
+                                   ax, ay, az, pot, nthreads);
+    free(rows);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Symmetric CPU twin: restatement of get_gravity_using_cpus                */
+/* (cpu_grav.f:1-381).  1-based Fortran index conventions at the interface. */
+/* Arrays have length ntot; outputs are fully overwritten (:17-22).         */
+/* ------------------------------------------------------------------------ */
+static void twin_rows(int ntot, int ilo, int ihi, int nkernel, int nselfgravity,
+                      const double *x, const double *y, const double *z,
+                      const double *am, const double *hp, const double *u,
+                      double *gx, double *gy, double *gz, double *grpot)
+{
+    /* 0-based internally: i in [ilo, ihi], j in [jlower, ntot-1] */
+    int jlower = (nselfgravity == 0) ? ntot - 1 : 0;   /* :24-30 */
+    const int jupper = ntot - 1;
+    const double c = self_coeff(nkernel);
+    for (int i = ilo; i <= ihi; ++i) {
+        if (i == ntot - 1) jlower = 0;                  /* :35 */
+        if (nselfgravity == 1) jlower = i;              /* :37 */
+        const double ami = am[i];
+        const double twohpi = 2 * hp[i];
+        const double fourhpi2 = twohpi * twohpi;
+        for (int j = jlower; j <= jupper; ++j) {
+            const double dr1 = x[j] - x[i], dr2 = y[j] - y[i], dr3 = z[j] - z[i];
+            const double r2 = dr1 * dr1 + dr2 * dr2 + dr3 * dr3;
+            const double rinv = 1 / sqrt(r2);
+            const double rinv2 = rinv * rinv;
+            const double amj = am[j];
+            const double twohpj = 2 * hp[j];
+            const double fourhpj2 = twohpj * twohpj;
+            const double massratio = ami / amj;
+            const double mrinv1 = rinv * amj;
+            const double mrinv3 = rinv2 * mrinv1;
+            double dgx, dgy, dgz, gpot;
+            if (r2 >= fmax(fourhpi2, fourhpj2)) {
+                dgx = mrinv3 * dr1; dgy = mrinv3 * dr2; dgz = mrinv3 * dr3;
+                gpot = -mrinv1;
+            } else {
+                double invq1, invq2, qq1, qq2;
+                if (r2 > 0) {
+                    invq1 = rinv * twohpi; invq2 = rinv * twohpj;
+                    qq1 = 1 / invq1; qq2 = 1 / invq2;
+                } else {
+                    invq1 = invq2 = qq1 = qq2 = 0.0;
+                }
+                double acc1, pot1, acc2, pot2;
+                kernel_AP(nkernel, qq1, invq1, &acc1, &pot1);
+                kernel_AP(nkernel, qq2, invq2, &acc2, &pot2);
+                const double mj11 = amj / twohpi, mj12 = amj / twohpj;
+                const double mj21 = mj11 / fourhpi2, mj22 = mj12 / fourhpj2;
+                const double g1 = (r2 <= fourhpi2) ? 1.0 : 0.0;
+                const double g2 = (r2 <= fourhpj2) ? 1.0 : 0.0;
+                if (nkernel != 0) {                     /* :224-242, :333-349 */
+                    if (g1 == 0.0) pot1 = 0.0;
+                    if (g2 == 0.0) pot2 = 0.0;
+                }
+                double gacc;
+                if (r2 > 0) {
+                    gacc = 0.5 * (g1 * mj21 * acc1 + (1.0 - g1) * mrinv3
+                                  + g2 * mj22 * acc2 + (1.0 - g2) * mrinv3);
+                    if (nkernel == 0)
+                        gpot = 0.5 * (g1 * mj11 * pot1 + (g1 - 1.0) * mrinv1
+                                      + g2 * mj12 * pot2 + (g2 - 1.0) * mrinv1);
+                    else
+                        gpot = 0.5 * (mj11 * pot1 + (g1 - 1.0) * mrinv1
+                                      + mj12 * pot2 + (g2 - 1.0) * mrinv1);
+                } else {
+                    gacc = 0.0;
+                    gpot = (u[i] != 0) ? -0.5 * c * (mj11 + mj12) : 0.0;   /* :140-144 */
+                }
+                dgx = gacc * dr1; dgy = gacc * dr2; dgz = gacc * dr3;
+            }
+            gx[i] += dgx; gx[j] -= dgx * massratio;
+            gy[i] += dgy; gy[j] -= dgy * massratio;
+            gz[i] += dgz; gz[j] -= dgz * massratio;
+            grpot[i] += gpot; grpot[j] += gpot * massratio;
+        }
+    }
+}
+
+/* Serial restatement: one "rank" owning rows ngrav_lower..ngrav_upper
+ * (1-based, inclusive), as cpu_grav.f:33. */
+int sgo_cpu_twin(int ntot, int ngrav_lower, int ngrav_upper, int nkernel, int nselfgravity,
+                 const double *x, const double *y, const double *z,
+                 const double *am, const double *hp, const double *u,
+                 double *gx, double *gy, double *gz, double *grpot)
+{
+    if (ntot <= 0 || ngrav_lower < 1 || ngrav_upper > ntot || !check_kernel(nkernel)) return -1;
+    memset(gx, 0, sizeof(double) * ntot); memset(gy, 0, sizeof(double) * ntot);
+    memset(gz, 0, sizeof(double) * ntot); memset(grpot, 0, sizeof(double) * ntot);
+    if (nselfgravity == 0 && u[ntot - 1] != 0.0) return -2;   /* :26-29 */
+    twin_rows(ntot, ngrav_lower - 1, ngrav_upper - 1, nkernel, nselfgravity,
+              x, y, z, am, hp, u, gx, gy, gz, grpot);
+    return 0;
+}
+
+/*
+ * Interaction-balanced split of the triangular j>=i loop over nranks
+ * (main.f:297-349).  lower/upper receive 1-based inclusive bounds per rank.
+ * Returns 0, or -1 when the reference's assignment fails (main.f:341-344).
+ * As in the reference, nranks == 1 fails: the loop assigns rank 0 at i = n,
+ * leaving irank = 1 != nranks - 1.  (n - i)/2 is integer division.
+ */
+int sgo_partition(int n, int nranks, int *lower, int *upper)
+{
+    if (n <= 0 || nranks <= 0) return -1;
+    const double tot = n * (n + 1.0) / 2;
+    int irank = 0;
+    int displ = 0;
+    double numint = 0;
+    for (int i = 1; i <= n; ++i) {
+        numint += n - i + 1;
+        if (numint + (n - i) / 2 >= (irank + 1.0) * tot / nranks) {
+            if (irank >= nranks) return -1;
+            lower[irank] = displ + 1;
+            upper[irank] = i;
+            irank += 1;
+            displ = i;
+            if (irank == nranks - 1) break;
+        }
+    }
+    if (irank != nranks - 1) return -1;
+    lower[irank] = displ + 1;
+    upper[irank] = n;
+    return 0;
+}
+
+/* OpenMP form of the twin: each thread plays one rank of sgo_partition with
+ * private accumulators, then a reduction (balAV3.f:436-442 mpi_reduce). */
+int sgo_cpu_twin_parallel(int ntot, int nkernel,
+                          const double *x, const double *y, const double *z,
+                          const double *am, const double *hp, const double *u,
+                          double *gx, double *gy, double *gz, double *grpot, int nthreads)
+{
+    if (ntot <= 0 || !check_kernel(nkernel)) return -1;
+    if (nthreads <= 0) nthreads = 1;
+    int *lo = (int *)malloc(sizeof(int) * nthreads), *hi = (int *)malloc(sizeof(int) * nthreads);
+    double *priv = (double *)calloc((size_t)nthreads * 4 * (size_t)ntot, sizeof(double));
+    if (!lo || !hi || !priv) { free(lo); free(hi); free(priv); return -1; }
+    if (nthreads == 1) {
+        lo[0] = 1;
+        hi[0] = ntot;
+    } else if (sgo_partition(ntot, nthreads, lo, hi) != 0) {
+        free(lo); free(hi); free(priv);
+        return -1;
+    }
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1)
+#endif
+    for (int t = 0; t < nthreads; ++t) {
+        double *p = priv + (size_t)t * 4 * ntot;
+        if (lo[t] <= hi[t])
+            twin_rows(ntot, lo[t] - 1, hi[t] - 1, nkernel, 1, x, y, z, am, hp, u,
+                      p, p + ntot, p + 2 * (size_t)ntot, p + 3 * (size_t)ntot);
+    }
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < ntot; ++i) {
+        double sx = 0, sy = 0, sz = 0, sp = 0;
+        for (int t = 0; t < nthreads; ++t) {
+            const double *p = priv + (size_t)t * 4 * ntot;
+            sx += p[i]; sy += p[ntot + i]; sz += p[2 * (size_t)ntot + i]; sp += p[3 * (size_t)ntot + i];
+        }
+        gx[i] = sx; gy[i] = sy; gz[i] = sz; grpot[i] = sp;
+    }
+    free(lo); free(hi); free(priv);
+    return 0;
+}
+
+/* Time-bounded sample of the twin for the CPU baseline: run rows
+ * [row_lo, row_hi] (0-based) of the symmetric loop with nselfgravity=1 and
+ * return the number of symmetric pair evaluations done. */
+int64_t sgo_cpu_twin_rows(int ntot, int row_lo, int row_hi, int nkernel,
+                          const double *x, const double *y, const double *z,
+                          const double *am, const double *hp, const double *u,
+                          double *gx, double *gy, double *gz, double *grpot)
+{
+    if (ntot <= 0 || row_lo < 0 || row_hi >= ntot || row_lo > row_hi || !check_kernel(nkernel))
+        return -1;
+    twin_rows(ntot, row_lo, row_hi, nkernel, 1, x, y, z, am, hp, u, gx, gy, gz, grpot);
+    int64_t evals = 0;
+    for (int64_t i = row_lo; i <= row_hi; ++i) evals += ntot - i;
+    return evals;
+}
+
+/* The same sample spread over nthreads OpenMP threads: rows split into
+ * chunks of equal pair count (as main.f:297-349 splits ranks), private
+ * accumulators, then a reduction.  Returns the symmetric evaluations done. */
+int64_t sgo_cpu_twin_rows_parallel(int ntot, int row_lo, int row_hi, int nkernel,
+                                   const double *x, const double *y, const double *z,
+                                   const double *am, const double *hp, const double *u,
+                                   double *gx, double *gy, double *gz, double *grpot,
+                                   int nthreads)
+{
+    if (ntot <= 0 || row_lo < 0 || row_hi >= ntot || row_lo > row_hi || !check_kernel(nkernel))
+        return -1;
+    if (nthreads <= 0) nthreads = 1;
+    int64_t total = 0;
+    for (int64_t i = row_lo; i <= row_hi; ++i) total += ntot - i;
+    int *lo = (int *)malloc(sizeof(int) * nthreads), *hi = (int *)malloc(sizeof(int) * nthreads);
+    double *priv = (double *)calloc((size_t)nthreads * 4 * (size_t)ntot, sizeof(double));
+    if (!lo || !hi || !priv) { free(lo); free(hi); free(priv); return -1; }
+    int t = 0;
+    int64_t acc = 0;
+    lo[0] = row_lo;
+    for (int i = row_lo; i <= row_hi && t < nthreads - 1; ++i) {
+        acc += ntot - i;
+        if (acc >= (t + 1) * (total / nthreads)) {
+            hi[t] = i;
+            lo[++t] = i + 1;
+        }
+    }
+    hi[t] = row_hi;
+    for (int k = t + 1; k < nthreads; ++k) { lo[k] = 1; hi[k] = 0; }   /* empty */
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1)
+#endif
+    for (int k = 0; k < nthreads; ++k) {
+        double *p = priv + (size_t)k * 4 * ntot;
+        if (lo[k] <= hi[k])
+            twin_rows(ntot, lo[k], hi[k], nkernel, 1, x, y, z, am, hp, u,
+                      p, p + ntot, p + 2 * (size_t)ntot, p + 3 * (size_t)ntot);
+    }
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < ntot; ++i) {
+        double s[4] = {0, 0, 0, 0};
+        for (int k = 0; k < nthreads; ++k)
+            for (int c = 0; c < 4; ++c) s[c] += priv[(size_t)k * 4 * ntot + (size_t)c * ntot + i];
+        gx[i] += s[0]; gy[i] += s[1]; gz[i] += s[2]; grpot[i] += s[3];
+    }
+    free(lo); free(hi); free(priv);
+    return total;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint (vector-Jacobian product) in FP64.                               */
+/* Loss L = sum_i w_i . a_i + u_i phi_i with the "limit" self rule.          */
+/* Per side k of a pair: F(r,H) = A(q)/H^3, Pp = P(q)/H if r < H (q = r/H),  */
+/* else F = r^-3, Pp = -1/r.  a_i = sum_j m_j 0.5(F_i+F_j) dr, and           */
+/* phi_i = sum_j m_j 0.5(Pp_i+Pp_j).  Gradients w.r.t. x, m and H (support   */
+/* radius, H = 2 hp) are returned.                                           */
+/* ------------------------------------------------------------------------ */
+typedef struct { double F, Pp, dF_dr, dP_dr, dF_dH, dP_dH; } side_t;
+
+static void side_eval(int nkernel, double r, double r2, double H, side_t *s)
+{
+    if (r2 <= H * H) {      /* softened side (same g rule as pair_direct) */
+        const double q = r / H;
+        double A, P, dA, dP;
+        kernel_AP(nkernel, q, q > 0 ? 1.0 / q : 0.0, &A, &P);
+        kernel_dAP(nkernel, q, &dA, &dP);
+        const double H2 = H * H, H3 = H2 * H, H4 = H2 * H2;
+        s->F = A / H3;
+        s->Pp = P / H;
+        s->dF_dr = dA / H4;
+        s->dP_dr = dP / H2;
+        s->dF_dH = -(3.0 * A + q * dA) / H4;
+        s->dP_dH = -(P + q * dP) / H2;
+    } else {
+        const double rinv = 1.0 / r;
+        s->F = rinv * rinv * rinv;
+        s->Pp = -rinv;
+        s->dF_dr = -3.0 * s->F * rinv;
+        s->dP_dr = rinv * rinv;
+        s->dF_dH = 0.0;
+        s->dP_dH = 0.0;
+    }
+}
+
+/* Contribution of ordered pair (i <- j) to L, and its partial derivatives.
+ * dL/d(dr) vector, dL/dm_j, dL/dH_i, dL/dH_j.  Self pair handled apart. */
+static void pair_loss_grad(int nkernel, double dx, double dy, double dz,
+                           double Hi, double Hj, double mj,
+                           const double wi[3], double ui,
+                           double *L, double g_dr[3], double *g_mj, double *g_Hi, double *g_Hj)
+{
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = sqrt(r2);
+    side_t si, sj;
+    if (r2 >= fmax(Hi * Hi, Hj * Hj)) {
+        side_eval(nkernel, r, r2, 0.0, &si);   /* H=0 forces the Newtonian side */
+        sj = si;
+    } else {
+        side_eval(nkernel, r, r2, Hi, &si);
+        side_eval(nkernel, r, r2, Hj, &sj);
+    }
+    const double F = 0.5 * (si.F + sj.F), Pp = 0.5 * (si.Pp + sj.Pp);
+    const double dF_dr = 0.5 * (si.dF_dr + sj.dF_dr), dP_dr = 0.5 * (si.dP_dr + sj.dP_dr);
+    const double wdr = wi[0] * dx + wi[1] * dy + wi[2] * dz;
+    *L = mj * (F * wdr + ui * Pp);
+    *g_mj = F * wdr + ui * Pp;
+    /* d/d(dr) of m_j [F(r) w.dr + u Pp(r)] = m_j [F w + (dF/dr w.dr + u dPp/dr) dr/r] */
+    const double radial = (dF_dr * wdr + ui * dP_dr) / r;
+    g_dr[0] = mj * (F * wi[0] + radial * dx);
+    g_dr[1] = mj * (F * wi[1] + radial * dy);
+    g_dr[2] = mj * (F * wi[2] + radial * dz);
+    *g_Hi = mj * 0.5 * (si.dF_dH * wdr + ui * si.dP_dH);
+    *g_Hj = mj * 0.5 * (sj.dF_dH * wdr + ui * sj.dP_dH);
+}
+
+/*
+ * Full gradient of L.  O(N^2) gather form: for every particle k, sum over the
+ * pairs where k is the i-side and where k is the j-side.  gx..gH have length n.
+ * Self pairs (i == j) contribute phi_self = -c flag (2 m / H): dL/dm and dL/dH.
+ * Distinct coincident pairs (r == 0) contribute their r->0 limit.
+ */
+int sgo_vjp(int n, const double *x, const double *y, const double *z,
+            const double *m, const double *range2, int nkernel,
+            const double *wx, const double *wy, const double *wz, const double *u,
+            double *gxo, double *gyo, double *gzo, double *gmo, double *gHo, int nthreads)
+{
+    if (n < 0 || !check_kernel(nkernel)) return -1;
+    const double c = self_coeff(nkernel);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t k = 0; k < n; ++k) {
+        const double Hk = decode_H(range2[k]);
+        double gx = 0, gy = 0, gz = 0, gm = 0, gH = 0;
+        for (int64_t l = 0; l < n; ++l) {
+            const double Hl = decode_H(range2[l]);
+            const double dx = x[l] - x[k], dy = y[l] - y[k], dz = z[l] - z[k];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0) {
+                /* potential only: pot_i += -c * s * (m_j/H_i + m_j/H_j), s = flag_j if i==j else 1 */
+                /* k as i-side (j = l): */
+                const double sl = (l == k) ? decode_flag(range2[l]) : 1.0;
+                const double sk = (l == k) ? decode_flag(range2[k]) : 1.0;
+                /* k is i, l is j */
+                gH += u[k] * (-c * sl) * (-m[l] / (Hk * Hk)) * (l == k ? 2.0 : 1.0);
+                if (l != k) {
+                    /* k is j, l is i: pot_l += -c*(m_k/H_l + m_k/H_k) */
+                    gm += u[l] * (-c * sk) * (1.0 / Hl + 1.0 / Hk);
+                    gH += u[l] * (-c * sk) * (-m[k] / (Hk * Hk));
+                } else {
+                    gm += u[k] * (-c * sk) * (2.0 / Hk);
+                }
+                continue;
+            }
+            double L, gdr[3], gmj, gHi, gHj;
+            /* k as i-side, l as j-side: dr = x_l - x_k, d(dr)/dx_k = -1 */
+            const double wk[3] = {wx[k], wy[k], wz[k]};
+            pair_loss_grad(nkernel, dx, dy, dz, Hk, Hl, m[l], wk, u[k], &L, gdr, &gmj, &gHi, &gHj);
+            gx -= gdr[0]; gy -= gdr[1]; gz -= gdr[2];
+            gH += gHi;
+            /* k as j-side, l as i-side: dr' = x_k - x_l = -dr, d(dr')/dx_k = +1 */
+            const double wl[3] = {wx[l], wy[l], wz[l]};
+            pair_loss_grad(nkernel, -dx, -dy, -dz, Hl, Hk, m[k], wl, u[l], &L, gdr, &gmj, &gHi, &gHj);
+            gx += gdr[0]; gy += gdr[1]; gz += gdr[2];
+            gm += gmj;
+            gH += gHj;
+        }
+        gxo[k] = gx; gyo[k] = gy; gzo[k] = gz; gmo[k] = gm; gHo[k] = gH;
+    }
+    return 0;
+}
+
+/* The part of L that depends on particle k's (x, m, H): its own row plus its
+ * column.  Finite differences of this against sgo_vjp pin the adjoint. */
+double sgo_loss_particle(int n, int k, const double *x, const double *y, const double *z,
+                         const double *m, const double *range2, int nkernel,
+                         const double *wx, const double *wy, const double *wz, const double *u)
+{
+    if (k < 0 || k >= n || !check_kernel(nkernel)) return NAN;
+    double L = 0.0;
+    for (int64_t l = 0; l < n; ++l) {
+        pairsum row = {0, 0, 0, 0}, col = {0, 0, 0, 0};
+        /* row: i = k, j = l */
+        pair_direct(nkernel, 0, l == k, x[l] - x[k], y[l] - y[k], z[l] - z[k],
+                    decode_H(range2[k]), decode_H(range2[l]), m[k], m[l],
+                    decode_flag(range2[l]), &row);
+        L += wx[k] * row.ax + wy[k] * row.ay + wz[k] * row.az + u[k] * row.pot;
+        if (l == k) continue;
+        /* column: i = l, j = k */
+        pair_direct(nkernel, 0, 0, x[k] - x[l], y[k] - y[l], z[k] - z[l],
+                    decode_H(range2[l]), decode_H(range2[k]), m[l], m[k],
+                    decode_flag(range2[k]), &col);
+        L += wx[l] * col.ax + wy[l] * col.ay + wz[l] * col.az + u[l] * col.pot;
+    }
+    return L;
+}
+
+/* Total-energy diagnostic epot = 1/2 sum m_i phi_i (output.f:232-252). */
+double sgo_epot(int64_t n, const double *m, const double *pot)
+{
+    double e = 0.0;
+    for (int64_t i = 0; i < n; ++i) e += m[i] * pot[i];
+    return 0.5 * e;
+}

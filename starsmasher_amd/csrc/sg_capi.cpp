@@ -1,0 +1,641 @@
+// sg_capi.cpp -- host layer of libsphgrav_hip.so: contexts, device buffer
+// management, the sg_* status-code API and the reference's three
+// Fortran-callable symbols (grav_force_direct.cu:648-677).
+//
+// Differences from the reference host code (grav_force_direct.cu:504-646),
+// all by design:
+//   * device buffers grow on demand and are reused across calls (the
+//     reference frees and re-allocates both buffers on every call,
+//     cuVector.h:42-59);
+//   * errors are status codes; the legacy symbols turn them into the
+//     reference's exit-on-error unless SPHGRAV_NOABORT=1;
+//   * the packing is done on the device (sg_prep.hip), not by a host loop.
+#include "sphgrav.h"
+#include "sg_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+using sg::TJ;
+
+struct sg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;       // stream work is enqueued on
+    hipStream_t own_stream = nullptr;   // the context's own stream
+    // events on the context stream: start, before far, after far, end
+    hipEvent_t ev_p0 = nullptr, ev_k0 = nullptr, ev_kf = nullptr, ev_k1 = nullptr;
+
+    // capacities (elements)
+    int64_t cap_n = 0;       // raw-set sized buffers (keys, vals, red)
+    int64_t cap_pad = 0;     // sorted j arrays
+    int64_t cap_ni = 0;      // sorted i arrays
+    int64_t cap_part = 0;    // j-partition partials (doubles)
+    int64_t cap_in = 0;      // host-path input staging
+    int64_t cap_out = 0;     // host-path output staging
+    size_t sort_tmp_bytes = 0;
+
+    sg::PrepBuffers pb{};
+    void *sort_tmp = nullptr;
+    double *partial = nullptr;
+    unsigned char *nearflag = nullptr;
+    int64_t cap_flag = 0;
+    unsigned long long *stats = nullptr;   // [2]
+    double *in[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *out[4] = {nullptr, nullptr, nullptr, nullptr};
+
+    int wg[2] = {64, 256};
+    int ipl[2] = {2, 1};
+    int jsplit[2] = {0, 0};
+
+    bool launched = false;
+};
+
+namespace {
+
+std::mutex g_mu;
+int g_last_status = SG_OK;
+
+#define HIP_TRY(expr)                                  \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) {                        \
+            last_hip_error() = e_;                     \
+            return SG_ERR_HIP;                         \
+        }                                              \
+    } while (0)
+
+hipError_t &last_hip_error()
+{
+    static thread_local hipError_t e = hipSuccess;
+    return e;
+}
+
+template <typename T>
+int grow(T **p, int64_t *cap, int64_t need)
+{
+    if (need <= *cap && *p) return SG_OK;
+    if (*p) HIP_TRY(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    const int64_t n = std::max<int64_t>(need, 1);
+    HIP_TRY(hipMalloc((void **)p, sizeof(T) * (size_t)n));
+    *cap = n;
+    return SG_OK;
+}
+
+// Capacity for a set of n particles (j) and ni i-particles.
+int ensure_capacity(sg_context *c, int64_t n, int64_t ni)
+{
+    const int64_t padded = ((n + TJ - 1) / TJ) * TJ;
+    if (n > c->cap_n || !c->pb.keys_in) {
+        int64_t cap = 0;
+        const int64_t need = std::max<int64_t>(n, 1);
+        int rc;
+        if ((rc = grow(&c->pb.keys_in, &cap, need)) != SG_OK) return rc;
+        cap = 0;
+        if ((rc = grow(&c->pb.keys_out, &cap, need)) != SG_OK) return rc;
+        cap = 0;
+        if ((rc = grow(&c->pb.vals_in, &cap, need)) != SG_OK) return rc;
+        cap = 0;
+        if ((rc = grow(&c->pb.vals_out, &cap, need)) != SG_OK) return rc;
+        size_t bytes = 0;
+        HIP_TRY(sg::sort_pairs_temp_bytes((int)need, &bytes));
+        if (bytes > c->sort_tmp_bytes || !c->sort_tmp) {
+            if (c->sort_tmp) HIP_TRY(hipFree(c->sort_tmp));
+            c->sort_tmp = nullptr;
+            HIP_TRY(hipMalloc(&c->sort_tmp, std::max<size_t>(bytes, 256)));
+            c->sort_tmp_bytes = bytes;
+        }
+        c->cap_n = need;
+    }
+    if (!c->pb.red) {
+        c->pb.nred_blocks = 1024;
+        int64_t cap = 0;
+        int rc;
+        if ((rc = grow(&c->pb.red, &cap, (int64_t)c->pb.nred_blocks * 8)) != SG_OK) return rc;
+        cap = 0;
+        if ((rc = grow(&c->pb.bbox, &cap, 8)) != SG_OK) return rc;
+        cap = 0;
+        if ((rc = grow(&c->pb.err, &cap, 4)) != SG_OK) return rc;
+        cap = 0;
+        if ((rc = grow(&c->stats, &cap, 2)) != SG_OK) return rc;
+    }
+    if (padded > c->cap_pad || !c->pb.sx) {
+        int64_t cap = 0;
+        int rc;
+        double **arrs[5] = {&c->pb.sx, &c->pb.sy, &c->pb.sz, &c->pb.sm, &c->pb.sH};
+        for (auto a : arrs) {
+            cap = 0;
+            if ((rc = grow(a, &cap, padded)) != SG_OK) return rc;
+        }
+        cap = 0;
+        if ((rc = grow(&c->pb.sidx, &cap, padded)) != SG_OK) return rc;
+        cap = 0;
+        if ((rc = grow(&c->pb.tiles, &cap, std::max<int64_t>(padded / TJ, 1))) != SG_OK) return rc;
+        c->cap_pad = padded;
+    }
+    if (ni > c->cap_ni || !c->pb.ix) {
+        int64_t cap = 0;
+        int rc;
+        const int64_t need = std::max<int64_t>(ni, 1);
+        double **arrs[5] = {&c->pb.ix, &c->pb.iy, &c->pb.iz, &c->pb.im, &c->pb.iH};
+        for (auto a : arrs) {
+            cap = 0;
+            if ((rc = grow(a, &cap, need)) != SG_OK) return rc;
+        }
+        cap = 0;
+        if ((rc = grow(&c->pb.iidx, &cap, need)) != SG_OK) return rc;
+        c->cap_ni = need;
+    }
+    return SG_OK;
+}
+
+int auto_jsplit(int niblocks, int wg, int ntiles)
+{
+    // aim for >= 4096 resident waves (256 CUs x 4 SIMDs x 4) when i alone
+    // cannot provide them; never more partitions than tiles
+    const int64_t waves = (int64_t)niblocks * (wg / 64);
+    if (waves >= 4096) return 1;
+    const int64_t want = (4096 + waves - 1) / waves;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(want, std::max(1, ntiles / 2)));
+}
+
+bool env_flag(const char *name)
+{
+    const char *v = std::getenv(name);
+    return v && *v && std::strcmp(v, "0") != 0;
+}
+
+}  // namespace
+
+// ============================================================ sg_* API
+
+extern "C" {
+
+const char *sg_status_string(int status)
+{
+    switch (status) {
+    case SG_OK: return "ok";
+    case SG_ERR_ARG: return "invalid argument (size, range or null pointer)";
+    case SG_ERR_NKERNEL: return "nkernel must be 0 (cubic), 1 (Wendland W3,3) or 2 (Wendland C4)";
+    case SG_ERR_MASS: return "particle mass <= 0";
+    case SG_ERR_RANGE2: return "range2 == 0 (zero smoothing length)";
+    case SG_ERR_NONFINITE: return "NaN or Inf in positions, masses or range2";
+    case SG_ERR_HIP: return hipGetErrorString(last_hip_error());
+    case SG_ERR_NO_DEVICE: return "no usable HIP device";
+    case SG_ERR_STATE: return "lasthalf called without a pending firsthalf";
+    case SG_ERR_CONFIG: return "unsupported tile configuration";
+    default: return "unknown status";
+    }
+}
+
+int sg_last_status(void) { return g_last_status; }
+
+int sg_device_count(int *count)
+{
+    if (!count) return SG_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return n > 0 ? SG_OK : SG_ERR_NO_DEVICE;
+}
+
+int sg_create(int device, sg_context **out)
+{
+    if (!out) return SG_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SG_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return SG_ERR_NO_DEVICE;
+    HIP_TRY(hipSetDevice(device));
+    sg_context *c = new sg_context();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev_p0) != hipSuccess || hipEventCreate(&c->ev_k0) != hipSuccess ||
+        hipEventCreate(&c->ev_kf) != hipSuccess || hipEventCreate(&c->ev_k1) != hipSuccess) {
+        delete c;
+        return SG_ERR_HIP;
+    }
+    c->stream = c->own_stream;
+    *out = c;
+    return SG_OK;
+}
+
+int sg_set_stream(sg_context *c, void *stream)
+{
+    if (!c) return SG_ERR_ARG;
+    HIP_TRY(hipSetDevice(c->device));
+    // work already queued on the previous stream completes first
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    return SG_OK;
+}
+
+int sg_destroy(sg_context *c)
+{
+    if (!c) return SG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->pb.red, c->pb.bbox, c->pb.err, c->pb.keys_in, c->pb.keys_out,
+                    c->pb.vals_in, c->pb.vals_out, c->pb.sx, c->pb.sy, c->pb.sz, c->pb.sm,
+                    c->pb.sH, c->pb.sidx, c->pb.tiles, c->pb.ix, c->pb.iy, c->pb.iz,
+                    c->pb.im, c->pb.iH, c->pb.iidx, c->sort_tmp, c->partial, c->nearflag, c->stats,
+                    c->in[0], c->in[1], c->in[2], c->in[3], c->in[4],
+                    c->out[0], c->out[1], c->out[2], c->out[3]};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    (void)hipEventDestroy(c->ev_p0);
+    (void)hipEventDestroy(c->ev_k0);
+    (void)hipEventDestroy(c->ev_kf);
+    (void)hipEventDestroy(c->ev_k1);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return SG_OK;
+}
+
+int sg_set_config(sg_context *c, int precision, int wg, int ipl, int jsplit)
+{
+    if (!c || (precision != SG_PREC_FP32C && precision != SG_PREC_FP64) || jsplit < 0)
+        return SG_ERR_ARG;
+    if (!sg::pair_config_supported(precision, wg, ipl)) return SG_ERR_CONFIG;
+    c->wg[precision] = wg;
+    c->ipl[precision] = ipl;
+    c->jsplit[precision] = jsplit;
+    return SG_OK;
+}
+
+int sg_get_config(sg_context *c, int precision, int *wg, int *ipl, int *jsplit)
+{
+    if (!c || (precision != SG_PREC_FP32C && precision != SG_PREC_FP64)) return SG_ERR_ARG;
+    if (wg) *wg = c->wg[precision];
+    if (ipl) *ipl = c->ipl[precision];
+    if (jsplit) *jsplit = c->jsplit[precision];
+    return SG_OK;
+}
+
+int sg_compute_device(sg_context *c, int n, int ibeg, int ni, const double *x, const double *y,
+                      const double *z, const double *m, const double *range2, int nkernel,
+                      int precision, double *ax, double *ay, double *az, double *pot)
+{
+    if (!c) return SG_ERR_ARG;
+    if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
+    if (precision != SG_PREC_FP32C && precision != SG_PREC_FP64) return SG_ERR_ARG;
+    if (n < 1 || ibeg < 0 || ni < 0 || (int64_t)ibeg + ni > n) return SG_ERR_ARG;
+    if (!x || !y || !z || !m || !range2) return SG_ERR_ARG;
+    if (ni > 0 && (!ax || !ay || !az || !pot)) return SG_ERR_ARG;
+    HIP_TRY(hipSetDevice(c->device));
+    c->launched = false;
+    if (ni == 0) return SG_OK;
+
+    const int64_t padded = ((int64_t)(n) + TJ - 1) / TJ * TJ;
+    int rc = ensure_capacity(c, n, ni);
+    if (rc != SG_OK) return rc;
+    hipStream_t s = c->stream;
+    sg::PrepBuffers &b = c->pb;
+    HIP_TRY(hipEventRecord(c->ev_p0, s));
+
+    // 1-2: validate + bbox, Morton keys of all n
+    HIP_TRY(sg::launch_validate_bbox(x, y, z, m, range2, n, b, s));
+    HIP_TRY(sg::launch_morton(x, y, z, 0, n, b, s));
+    // 3: sort
+    HIP_TRY(sg::sort_pairs(c->sort_tmp, c->sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
+                           b.vals_out, n, s));
+    // 4-5: gather sorted j arrays, tile metadata
+    HIP_TRY(sg::launch_gather(x, y, z, m, range2, b.vals_out, n, padded, b.sx, b.sy, b.sz, b.sm,
+                              b.sH, b.sidx, s));
+    const int ntiles = (int)(padded / TJ);
+    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, ntiles, b.tiles, s));
+
+    sg::PairArgs a{};
+    a.j = sg::SortedSet{b.sx, b.sy, b.sz, b.sm, b.sH, b.sidx, n, padded};
+    if (ibeg == 0 && ni == n) {
+        a.i = a.j;
+    } else {
+        // the i-range gets its own curve order
+        HIP_TRY(sg::launch_morton(x, y, z, ibeg, ni, b, s));
+        HIP_TRY(sg::sort_pairs(c->sort_tmp, c->sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
+                               b.vals_out, ni, s));
+        HIP_TRY(sg::launch_gather(x, y, z, m, range2, b.vals_out, ni, ni, b.ix, b.iy, b.iz, b.im,
+                                  b.iH, b.iidx, s));
+        a.i = sg::SortedSet{b.ix, b.iy, b.iz, b.im, b.iH, b.iidx, ni, ni};
+    }
+    a.tiles = b.tiles;
+    a.ni = ni;
+    a.ibeg = ibeg;
+    a.nkernel = nkernel;
+    a.ntiles = ntiles;
+    const int wg = c->wg[precision], ipl = c->ipl[precision];
+    a.ipl = ipl;
+    a.niblocks = (int)((ni + (int64_t)wg * ipl - 1) / ((int64_t)wg * ipl));
+    a.ngroups = (int64_t)a.niblocks * (wg / 64);
+    a.jsplit = c->jsplit[precision] > 0 ? std::min(c->jsplit[precision], ntiles)
+                                        : auto_jsplit(a.niblocks, wg, ntiles);
+    a.ax = ax;
+    a.ay = ay;
+    a.az = az;
+    a.pot = pot;
+    if ((rc = grow(&c->partial, &c->cap_part, (int64_t)a.jsplit * 4 * ni)) != SG_OK) return rc;
+    a.partial = c->partial;
+    if ((rc = grow(&c->nearflag, &c->cap_flag, a.ngroups * ntiles)) != SG_OK) return rc;
+    a.nearflag = c->nearflag;
+    a.stats = c->stats;
+    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 4, s));
+
+    HIP_TRY(hipEventRecord(c->ev_k0, s));
+    HIP_TRY(sg::launch_pair_far(precision, a, wg, s));
+    HIP_TRY(hipEventRecord(c->ev_kf, s));
+    HIP_TRY(sg::launch_pair_near(a, s));
+    HIP_TRY(sg::launch_reduce_partials(a, s));
+    HIP_TRY(hipEventRecord(c->ev_k1, s));
+    c->launched = true;
+    return SG_OK;
+}
+
+int sg_synchronize(sg_context *c)
+{
+    if (!c) return SG_ERR_ARG;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!c->launched) return SG_OK;
+    int err = 0;
+    HIP_TRY(hipMemcpy(&err, c->pb.err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err & sg::ERR_NONFINITE) return SG_ERR_NONFINITE;
+    if (err & sg::ERR_MASS) return SG_ERR_MASS;
+    if (err & sg::ERR_RANGE2) return SG_ERR_RANGE2;
+    return SG_OK;
+}
+
+int sg_last_kernel_ms(sg_context *c, float *ms)
+{
+    if (!c || !ms) return SG_ERR_ARG;
+    if (!c->launched) return SG_ERR_STATE;
+    HIP_TRY(hipEventElapsedTime(ms, c->ev_k0, c->ev_k1));
+    return SG_OK;
+}
+
+int sg_last_timings(sg_context *c, float *prep_ms, float *far_ms, float *near_ms)
+{
+    if (!c) return SG_ERR_ARG;
+    if (!c->launched) return SG_ERR_STATE;
+    float t;
+    if (prep_ms) {
+        HIP_TRY(hipEventElapsedTime(&t, c->ev_p0, c->ev_k0));
+        *prep_ms = t;
+    }
+    if (far_ms) {
+        HIP_TRY(hipEventElapsedTime(&t, c->ev_k0, c->ev_kf));
+        *far_ms = t;
+    }
+    if (near_ms) {
+        HIP_TRY(hipEventElapsedTime(&t, c->ev_kf, c->ev_k1));
+        *near_ms = t;
+    }
+    return SG_OK;
+}
+
+int sg_last_tile_stats(sg_context *c, long long *visits, long long *near_visits)
+{
+    if (!c) return SG_ERR_ARG;
+    if (!c->launched) return SG_ERR_STATE;
+    unsigned long long h[2] = {0, 0};
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(h, c->stats, sizeof(h), hipMemcpyDeviceToHost));
+    if (visits) *visits = (long long)h[0];
+    if (near_visits) *near_visits = (long long)h[1];
+    return SG_OK;
+}
+
+int sg_compute_host(sg_context *c, int n, int ibeg, int ni, const double *x, const double *y,
+                    const double *z, const double *m, const double *range2, int nkernel,
+                    int precision, double *ax, double *ay, double *az, double *pot)
+{
+    if (!c) return SG_ERR_ARG;
+    if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
+    if (n < 1 || ibeg < 0 || ni < 0 || (int64_t)ibeg + ni > n) return SG_ERR_ARG;
+    if (!x || !y || !z || !m || !range2) return SG_ERR_ARG;
+    if (ni > 0 && (!ax || !ay || !az || !pot)) return SG_ERR_ARG;
+    HIP_TRY(hipSetDevice(c->device));
+    int rc;
+    for (int k = 0; k < 5; ++k) {
+        int64_t cap = c->cap_in;
+        if ((rc = grow(&c->in[k], &cap, n)) != SG_OK) return rc;
+    }
+    c->cap_in = std::max<int64_t>(c->cap_in, n);
+    for (int k = 0; k < 4; ++k) {
+        int64_t cap = c->cap_out;
+        if ((rc = grow(&c->out[k], &cap, std::max(ni, 1))) != SG_OK) return rc;
+    }
+    c->cap_out = std::max<int64_t>(c->cap_out, std::max(ni, 1));
+    const double *src[5] = {x, y, z, m, range2};
+    for (int k = 0; k < 5; ++k)
+        HIP_TRY(hipMemcpyAsync(c->in[k], src[k], sizeof(double) * (size_t)n, hipMemcpyHostToDevice,
+                               c->stream));
+    rc = sg_compute_device(c, n, ibeg, ni, c->in[0], c->in[1], c->in[2], c->in[3], c->in[4],
+                           nkernel, precision, c->out[0], c->out[1], c->out[2], c->out[3]);
+    if (rc != SG_OK) return rc;
+    rc = sg_synchronize(c);
+    if (rc != SG_OK) return rc;
+    double *dst[4] = {ax, ay, az, pot};
+    for (int k = 0; k < 4 && ni > 0; ++k)
+        HIP_TRY(hipMemcpy(dst[k], c->out[k], sizeof(double) * (size_t)ni, hipMemcpyDeviceToHost));
+    return SG_OK;
+}
+
+}  // extern "C"
+
+// ===================================================== legacy layer state
+namespace {
+
+struct Legacy {
+    int device = -1;
+    std::vector<sg_context *> ctx;   // one per device, created on first use
+    bool pending = false;
+    int ibeg = 0, ni = 0;
+    sg_context *cur = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+Legacy &legacy()
+{
+    static Legacy L;
+    return L;
+}
+
+int legacy_precision()
+{
+    const char *v = std::getenv("SPHGRAV_PRECISION");
+    return (v && (std::strcmp(v, "fp64") == 0 || std::strcmp(v, "FP64") == 0)) ? SG_PREC_FP64
+                                                                               : SG_PREC_FP32C;
+}
+
+int legacy_context(sg_context **out)
+{
+    Legacy &L = legacy();
+    int dev = L.device;
+    if (dev < 0) {
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return SG_ERR_NO_DEVICE;
+    if (dev >= count) return SG_ERR_NO_DEVICE;
+    if ((int)L.ctx.size() < count) L.ctx.resize(count, nullptr);
+    if (!L.ctx[dev]) {
+        int rc = sg_create(dev, &L.ctx[dev]);
+        if (rc != SG_OK) return rc;
+    }
+    if (!L.t0) {
+        HIP_TRY(hipSetDevice(dev));
+        HIP_TRY(hipEventCreate(&L.t0));
+        HIP_TRY(hipEventCreate(&L.t1));
+    }
+    *out = L.ctx[dev];
+    return SG_OK;
+}
+
+// Host-side argument checks of the reference's packing loop
+// (grav_force_direct.cu:597-602), turned into status codes.
+int check_host_inputs(int n, const double *x, const double *y, const double *z, const double *m,
+                      const double *r2)
+{
+    for (int k = 0; k < n; ++k) {
+        if (!(std::isfinite(x[k]) && std::isfinite(y[k]) && std::isfinite(z[k]) &&
+              std::isfinite(m[k]) && std::isfinite(r2[k])))
+            return SG_ERR_NONFINITE;
+        if (!(m[k] > 0.0)) return SG_ERR_MASS;
+        if (r2[k] == 0.0) return SG_ERR_RANGE2;
+    }
+    return SG_OK;
+}
+
+void legacy_fail(const char *where, int status)
+{
+    g_last_status = status;
+    if (status == SG_OK) return;
+    std::fprintf(stderr, " SPHgrav(%s): error %d: %s\n", where, status, sg_status_string(status));
+    if (!env_flag("SPHGRAV_NOABORT")) std::exit(EXIT_FAILURE);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sg_init_device(int device)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return SG_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return SG_ERR_NO_DEVICE;
+    HIP_TRY(hipSetDevice(device));
+    legacy().device = device;
+    return SG_OK;
+}
+
+int sg_firsthalf(int n, int n_lower, int my_length, const double *x, const double *y,
+                 const double *z, const double *m, const double *range2, int nkernel)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Legacy &L = legacy();
+    if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
+    if (n < 1 || n_lower < 1 || my_length < 0 || (int64_t)n_lower - 1 + my_length > n)
+        return SG_ERR_ARG;
+    if (!x || !y || !z || !m || !range2) return SG_ERR_ARG;
+    int rc = check_host_inputs(n, x, y, z, m, range2);
+    if (rc != SG_OK) return rc;
+    sg_context *c = nullptr;
+    if ((rc = legacy_context(&c)) != SG_OK) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if (L.pending) {
+        // the reference allows one outstanding call; a second firsthalf
+        // replaces the first (its results are dropped)
+        std::fprintf(stderr, " SPHgrav: firsthalf called twice; dropping the pending result\n");
+        HIP_TRY(hipStreamSynchronize(L.cur->stream));
+        L.pending = false;
+    }
+    const int ni = my_length;
+    for (int k = 0; k < 5; ++k) {
+        int64_t cap = c->cap_in;
+        if ((rc = grow(&c->in[k], &cap, n)) != SG_OK) return rc;
+    }
+    c->cap_in = std::max<int64_t>(c->cap_in, n);
+    for (int k = 0; k < 4; ++k) {
+        int64_t cap = c->cap_out;
+        if ((rc = grow(&c->out[k], &cap, std::max(ni, 1))) != SG_OK) return rc;
+    }
+    c->cap_out = std::max<int64_t>(c->cap_out, std::max(ni, 1));
+    HIP_TRY(hipEventRecord(L.t0, c->stream));
+    const double *src[5] = {x, y, z, m, range2};
+    for (int k = 0; k < 5; ++k)
+        HIP_TRY(hipMemcpyAsync(c->in[k], src[k], sizeof(double) * (size_t)n, hipMemcpyHostToDevice,
+                               c->stream));
+    // the host arrays may be reused as soon as firsthalf returns
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    rc = sg_compute_device(c, n, n_lower - 1, ni, c->in[0], c->in[1], c->in[2], c->in[3],
+                           c->in[4], nkernel, legacy_precision(), c->out[0], c->out[1], c->out[2],
+                           c->out[3]);
+    if (rc != SG_OK) return rc;
+    L.pending = true;
+    L.cur = c;
+    L.ibeg = n_lower - 1;
+    L.ni = ni;
+    return SG_OK;
+}
+
+int sg_lasthalf(double *ax, double *ay, double *az, double *pot)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Legacy &L = legacy();
+    if (!L.pending) return SG_ERR_STATE;
+    if (L.ni > 0 && (!ax || !ay || !az || !pot)) return SG_ERR_ARG;
+    sg_context *c = L.cur;
+    L.pending = false;
+    HIP_TRY(hipSetDevice(c->device));
+    int rc = sg_synchronize(c);
+    if (rc != SG_OK) return rc;
+    double *dst[4] = {ax, ay, az, pot};
+    for (int k = 0; k < 4 && L.ni > 0; ++k)
+        HIP_TRY(hipMemcpyAsync(dst[k] + L.ibeg, c->out[k], sizeof(double) * (size_t)L.ni,
+                               hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(L.t1, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!env_flag("SPHGRAV_QUIET")) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, L.t0, L.t1) == hipSuccess)
+            std::fprintf(stderr, " >>> SPHgrav_lib took %g sec\n", ms / 1000.0);
+    }
+    return SG_OK;
+}
+
+// ------------------------------------------------ reference ABI symbols
+
+void firsthalf_grav_forces_(int *n, int *n_lower, int *my_length, double *px, double *py,
+                            double *pz, double *mass, double *range2, int *q, int *nkernel)
+{
+    (void)q;
+    if (!n || !n_lower || !my_length || !nkernel) {
+        legacy_fail("firsthalf_grav_forces", SG_ERR_ARG);
+        return;
+    }
+    legacy_fail("firsthalf_grav_forces",
+                sg_firsthalf(*n, *n_lower, *my_length, px, py, pz, mass, range2, *nkernel));
+}
+
+void lasthalf_grav_forces_(int *n, double *ax, double *ay, double *az, double *pot, int *myrank)
+{
+    (void)n;
+    (void)myrank;
+    legacy_fail("lasthalf_grav_forces", sg_lasthalf(ax, ay, az, pot));
+}
+
+void gpu_init_dev_(int *device, double *theta)
+{
+    (void)theta;
+    legacy_fail("gpu_init_dev", device ? sg_init_device(*device) : SG_ERR_ARG);
+}
+
+}  // extern "C"

@@ -1,0 +1,551 @@
+// sg_direct.hip -- the O(N^2) pair kernels for gfx950 (CDNA4).
+//
+// Reference: dev_compute_forces<256> and dev_force_ij_nkernel{0,1,2}
+// (pbe/src/SPHgrav_lib2/grav_force_direct.cu:55-502): one i per thread, j in
+// 256-wide shared-memory tiles, every pair through the full three-way softened
+// branch, float-float (Fast2Sum) accumulation of every term.
+//
+// Design here (wave64, VALU-bound, no MFMA -- a pairwise sum is not a
+// contraction):
+//   * particles arrive sorted along a Morton curve (sg_prep.hip), so an
+//     i-group of 64*IPL consecutive particles (one wave) and a j-tile of TJ
+//     consecutive particles are both spatially compact;
+//   * FAR kernel: each wave holds IPL i-particles per lane in registers and
+//     tests every j-tile once against its own bounding box.  If the boxes are
+//     farther apart than the largest support radius of both, every pair in
+//     the tile is Newtonian (the grav_force_direct.cu:69-75 branch) and runs a
+//     branch-free 14-instruction body per pair (3 sub, 3 for r^2, rsq, 3 mul,
+//     3 fma, 1 add).  Otherwise the tile is flagged for the near kernel;
+//   * j data for far tiles is staged through LDS as FP32 {x_hi, y_hi, z_hi, m}
+//     and {x_lo, y_lo, z_lo} relative to an FP64 origin taken per workgroup,
+//     written from coalesced FP64 global loads and read back as same-address
+//     (broadcast) ds_read_b128.  Waves whose extent is large against their
+//     smallest H use hi+lo differences; compact waves use hi only;
+//   * FP32 partial sums are folded into FP64 accumulators after every LDS
+//     stage of TS j-particles (the cross-tile reduction is FP64);
+//   * NEAR kernel: the flagged (wave, tile) pairs, every pair through the full
+//     softened branch in FP64 (the Wendland polynomials carry coefficients up
+//     to 9009 that cancel to O(1) near q = 1, which FP32 cannot resolve);
+//   * REDUCE kernel: fixed-order FP64 sum over j-partitions, scattered back to
+//     input order.
+// The far kernel carries no FP64 softening code, so its register budget (and
+// occupancy) is set by the 14-instruction loop alone.
+#include "sg_internal.h"
+
+#include <math.h>
+
+namespace sg {
+namespace {
+
+constexpr int TS = 128;                 // j per LDS stage; TJ / TS stages per tile
+constexpr float FAR_MARGIN = 1.0e-3f;   // relative guard on the box-distance test
+constexpr float WIDE_RATIO = 8.0f;      // wave extent / min H above which dr uses hi+lo
+
+static_assert(TJ % TS == 0, "tile must be a whole number of stages");
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ float uniform_f(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return uniform_f(v);
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return uniform_f(v);
+}
+
+// Box-distance test of a wave's i-box [bl, bh] against tile tm, both
+// relative to the FP64 origin O.  True when every pair is at least the
+// larger support radius apart, i.e. every pair takes the Newtonian branch.
+__device__ __forceinline__ bool tile_is_far(const TileMeta &tm, double Ox, double Oy, double Oz,
+                                            const float (&bl)[3], const float (&bh)[3], float hmax)
+{
+    const float lx = (float)(tm.lo[0] - Ox), ly = (float)(tm.lo[1] - Oy), lz = (float)(tm.lo[2] - Oz);
+    const float ux = (float)(tm.hi[0] - Ox), uy = (float)(tm.hi[1] - Oy), uz = (float)(tm.hi[2] - Oz);
+    const float gx = fmaxf(0.0f, fmaxf(lx - bh[0], bl[0] - ux));
+    const float gy = fmaxf(0.0f, fmaxf(ly - bh[1], bl[1] - uy));
+    const float gz = fmaxf(0.0f, fmaxf(lz - bh[2], bl[2] - uz));
+    const float d2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+    const float hm = fmaxf(hmax, (float)tm.maxH);
+    return d2 > hm * hm * (1.0f + FAR_MARGIN);
+}
+
+__device__ __forceinline__ void tile_range(const PairArgs &a, int jp, int &t0, int &t1)
+{
+    t0 = (int)((int64_t)a.ntiles * jp / a.jsplit);
+    t1 = (int)((int64_t)a.ntiles * (jp + 1) / a.jsplit);
+}
+
+// Sorted-order FP64 partial sums of one i for j-partition jp.
+__device__ __forceinline__ double *partial_of(const PairArgs &a, int jp)
+{
+    return a.partial + (int64_t)jp * 4 * a.ni;
+}
+
+// ------------------------------------------------ far kernel, FP32 + comp.
+template <int IPL, bool WIDE>
+__device__ __forceinline__ void far_stage_fp32(const float4 *s_hi, const float4 *s_lo,
+                                               const float (&xh)[IPL], const float (&yh)[IPL],
+                                               const float (&zh)[IPL], const float (&xl)[IPL],
+                                               const float (&yl)[IPL], const float (&zl)[IPL],
+                                               float (&fx)[IPL], float (&fy)[IPL],
+                                               float (&fz)[IPL], float (&fp)[IPL])
+{
+#pragma unroll 4
+    for (int jj = 0; jj < TS; ++jj) {
+        const float4 p = s_hi[jj];
+        float4 q;
+        if (WIDE) q = s_lo[jj];
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            float dx = p.x - xh[k], dy = p.y - yh[k], dz = p.z - zh[k];
+            if (WIDE) {
+                dx += q.x - xl[k];
+                dy += q.y - yl[k];
+                dz += q.z - zl[k];
+            }
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float ri = __builtin_amdgcn_rsqf(r2);
+            const float mr = p.w * ri;
+            const float mr3 = mr * ri * ri;
+            fx[k] = fmaf(mr3, dx, fx[k]);
+            fy[k] = fmaf(mr3, dy, fy[k]);
+            fz[k] = fmaf(mr3, dz, fz[k]);
+            fp[k] += mr;
+        }
+    }
+}
+
+template <int WG, int IPL>
+__global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
+{
+    constexpr int WI = 64 * IPL;   // i per wave
+    constexpr int GI = WG * IPL;   // i per workgroup
+    __shared__ float4 s_hi[TS];
+    __shared__ float4 s_lo[TS];
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int ib = blockIdx.x % a.niblocks;
+    const int jp = blockIdx.x / a.niblocks;
+    int t0, t1;
+    tile_range(a, jp, t0, t1);
+    const int64_t group = (int64_t)ib * (WG / 64) + wv;
+
+    // FP64 origin of this workgroup: its middle i-particle.
+    const int64_t gbase = (int64_t)ib * GI;
+    const int64_t io = min(gbase + GI / 2, a.ni - 1);
+    const double Ox = a.i.x[io], Oy = a.i.y[io], Oz = a.i.z[io];
+
+    float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL];
+    int64_t ii[IPL];
+    bool valid[IPL];
+    float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float hmax = 0.0f, hmin = INFINITY;
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        const int64_t i = gbase + wv * WI + k * 64 + lane;
+        valid[k] = i < a.ni;
+        ii[k] = valid[k] ? i : a.ni - 1;
+        const double rx = a.i.x[ii[k]] - Ox, ry = a.i.y[ii[k]] - Oy, rz = a.i.z[ii[k]] - Oz;
+        xh[k] = (float)rx;
+        yh[k] = (float)ry;
+        zh[k] = (float)rz;
+        xl[k] = (float)(rx - (double)xh[k]);
+        yl[k] = (float)(ry - (double)yh[k]);
+        zl[k] = (float)(rz - (double)zh[k]);
+        const float H = (float)fabs(a.i.Hs[ii[k]]);
+        bl[0] = fminf(bl[0], xh[k]); bl[1] = fminf(bl[1], yh[k]); bl[2] = fminf(bl[2], zh[k]);
+        bh[0] = fmaxf(bh[0], xh[k]); bh[1] = fmaxf(bh[1], yh[k]); bh[2] = fmaxf(bh[2], zh[k]);
+        hmax = fmaxf(hmax, H);
+        hmin = fminf(hmin, H);
+    }
+    for (int c = 0; c < 3; ++c) {
+        bl[c] = wave_min(bl[c]);
+        bh[c] = wave_max(bh[c]);
+    }
+    hmax = wave_max(hmax);
+    hmin = wave_min(hmin);
+    const float ext = fmaxf(fmaxf(bh[0] - bl[0], bh[1] - bl[1]), bh[2] - bl[2]);
+    // |x - O| of i and j reaches ext + r: take hi+lo differences when that
+    // could cost more than ~1e-6 relative in dr
+    const bool wide = uniform_i(ext > WIDE_RATIO * hmin) != 0;
+
+    float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
+    double ax[IPL], ay[IPL], az[IPL], ap[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
+        ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+    }
+
+    unsigned nearv = 0;
+    unsigned char *flags = a.nearflag + group * a.ntiles;
+    for (int t = t0; t < t1; ++t) {
+        const bool far = uniform_i(tile_is_far(a.tiles[t], Ox, Oy, Oz, bl, bh, hmax)) != 0;
+        if (lane == 0) flags[t] = far ? 0 : 1;
+        nearv += far ? 0 : 1;
+        if (WG == 64 && !far) continue;
+        for (int s = 0; s < TJ / TS; ++s) {
+            __syncthreads();
+            for (int jj = tid; jj < TS; jj += WG) {
+                const int64_t j = (int64_t)t * TJ + s * TS + jj;
+                const double rx = a.j.x[j] - Ox, ry = a.j.y[j] - Oy, rz = a.j.z[j] - Oz;
+                const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+                s_hi[jj] = make_float4(hx, hy, hz, (float)a.j.m[j]);
+                s_lo[jj] = make_float4((float)(rx - (double)hx), (float)(ry - (double)hy),
+                                       (float)(rz - (double)hz), 0.0f);
+            }
+            __syncthreads();
+            if (far) {
+                if (wide)
+                    far_stage_fp32<IPL, true>(s_hi, s_lo, xh, yh, zh, xl, yl, zl, fx, fy, fz, fp);
+                else
+                    far_stage_fp32<IPL, false>(s_hi, s_lo, xh, yh, zh, xl, yl, zl, fx, fy, fz, fp);
+#pragma unroll
+                for (int k = 0; k < IPL; ++k) {
+                    ax[k] += (double)fx[k];
+                    ay[k] += (double)fy[k];
+                    az[k] += (double)fz[k];
+                    ap[k] -= (double)fp[k];
+                    fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
+                }
+            }
+        }
+    }
+
+    double *p = partial_of(a, jp);
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        if (!valid[k]) continue;
+        p[ii[k]] = ax[k];
+        p[a.ni + ii[k]] = ay[k];
+        p[2 * a.ni + ii[k]] = az[k];
+        p[3 * a.ni + ii[k]] = ap[k];
+    }
+    if (a.stats && lane == 0) {
+        atomicAdd(&a.stats[0], (unsigned long long)(t1 - t0));
+        atomicAdd(&a.stats[1], (unsigned long long)nearv);
+    }
+}
+
+// ------------------------------------------------------- far kernel, FP64
+template <int WG, int IPL>
+__global__ __launch_bounds__(WG) void k_far_fp64(PairArgs a)
+{
+    constexpr int WI = 64 * IPL;
+    constexpr int GI = WG * IPL;
+    __shared__ double4 s_p[TS];
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int ib = blockIdx.x % a.niblocks;
+    const int jp = blockIdx.x / a.niblocks;
+    int t0, t1;
+    tile_range(a, jp, t0, t1);
+    const int64_t group = (int64_t)ib * (WG / 64) + wv;
+
+    const int64_t gbase = (int64_t)ib * GI;
+    const int64_t io = min(gbase + GI / 2, a.ni - 1);
+    const double Ox = a.i.x[io], Oy = a.i.y[io], Oz = a.i.z[io];
+
+    double xr[IPL], yr[IPL], zr[IPL];
+    int64_t ii[IPL];
+    bool valid[IPL];
+    float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float hmax = 0.0f;
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        const int64_t i = gbase + wv * WI + k * 64 + lane;
+        valid[k] = i < a.ni;
+        ii[k] = valid[k] ? i : a.ni - 1;
+        xr[k] = a.i.x[ii[k]] - Ox;
+        yr[k] = a.i.y[ii[k]] - Oy;
+        zr[k] = a.i.z[ii[k]] - Oz;
+        // the same FP32 box as the FP32 kernel, so both classify tiles alike
+        bl[0] = fminf(bl[0], (float)xr[k]); bl[1] = fminf(bl[1], (float)yr[k]);
+        bl[2] = fminf(bl[2], (float)zr[k]);
+        bh[0] = fmaxf(bh[0], (float)xr[k]); bh[1] = fmaxf(bh[1], (float)yr[k]);
+        bh[2] = fmaxf(bh[2], (float)zr[k]);
+        hmax = fmaxf(hmax, (float)fabs(a.i.Hs[ii[k]]));
+    }
+    for (int c = 0; c < 3; ++c) {
+        bl[c] = wave_min(bl[c]);
+        bh[c] = wave_max(bh[c]);
+    }
+    hmax = wave_max(hmax);
+
+    double ax[IPL], ay[IPL], az[IPL], ap[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+
+    unsigned nearv = 0;
+    unsigned char *flags = a.nearflag + group * a.ntiles;
+    for (int t = t0; t < t1; ++t) {
+        const bool far = uniform_i(tile_is_far(a.tiles[t], Ox, Oy, Oz, bl, bh, hmax)) != 0;
+        if (lane == 0) flags[t] = far ? 0 : 1;
+        nearv += far ? 0 : 1;
+        if (WG == 64 && !far) continue;
+        for (int s = 0; s < TJ / TS; ++s) {
+            __syncthreads();
+            for (int jj = tid; jj < TS; jj += WG) {
+                const int64_t j = (int64_t)t * TJ + s * TS + jj;
+                s_p[jj] = make_double4(a.j.x[j] - Ox, a.j.y[j] - Oy, a.j.z[j] - Oz, a.j.m[j]);
+            }
+            __syncthreads();
+            if (!far) continue;
+#pragma unroll 2
+            for (int jj = 0; jj < TS; ++jj) {
+                const double4 q = s_p[jj];
+#pragma unroll
+                for (int k = 0; k < IPL; ++k) {
+                    const double dx = q.x - xr[k], dy = q.y - yr[k], dz = q.z - zr[k];
+                    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    const double ri = rsqrt(r2);
+                    const double mr = q.w * ri;
+                    const double mr3 = mr * ri * ri;
+                    ax[k] = fma(mr3, dx, ax[k]);
+                    ay[k] = fma(mr3, dy, ay[k]);
+                    az[k] = fma(mr3, dz, az[k]);
+                    ap[k] -= mr;
+                }
+            }
+        }
+    }
+
+    double *p = partial_of(a, jp);
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        if (!valid[k]) continue;
+        p[ii[k]] = ax[k];
+        p[a.ni + ii[k]] = ay[k];
+        p[2 * a.ni + ii[k]] = az[k];
+        p[3 * a.ni + ii[k]] = ap[k];
+    }
+    if (a.stats && lane == 0) {
+        atomicAdd(&a.stats[0], (unsigned long long)(t1 - t0));
+        atomicAdd(&a.stats[1], (unsigned long long)nearv);
+    }
+}
+
+// ------------------------------------------------- softened pair, FP64
+// Kernel polynomials A(q) (force) and P(q) (potential), q = r/H, with the
+// reference's decimal coefficients (cpu_grav.f:91-117, 210-243, 321-350).
+__device__ __forceinline__ void poly_AP(int nk, double q, double invq, double &A, double &P)
+{
+    const double q2 = q * q;
+    if (nk == 0) {
+        if (q < 0.5) {
+            A = 10.666666666667 + q2 * (32.0 * q - 38.4);
+            P = -2.8 + q2 * (5.333333333333 + q2 * (6.4 * q - 9.6));
+        } else {
+            const double iq3 = invq * invq * invq;
+            A = 21.333333333333 + q * (-48.0 + q * (38.4 - 10.666666666667 * q))
+                - 0.066666666667 * iq3;
+            P = -3.2 + 0.066666666667 * invq
+                + q2 * (10.666666666667 + q * (-16.0 + q * (9.6 - 2.133333333333 * q)));
+        }
+    } else if (nk == 1) {
+        A = 28.4375 + q2 * (-187.6875 + q2 * (804.375 + q2 * (-4379.375 + q * (9009.0
+            + q * (-8957.8125 + q * (5005.0 + q * (-1515.9375 + 195.0 * q)))))));
+        P = -3.828125 + q2 * (14.21875 + q2 * (-46.921875 + q2 * (134.0625 + q2 * (
+            -547.421875 + q * (1001.0 + q * (-895.78125 + q * (455.0 + q * (
+            -126.328125 + 15.0 * q))))))));
+    } else {
+        A = 20.625 + q2 * (-115.5 + q2 * (618.75 + q * (-1155.0 + q * (962.5
+            + q * (-396.0 + 65.625 * q)))));
+        P = -3.4375 + q2 * (10.3125 + q2 * (-28.875 + q2 * (103.125 + q * (-165.0
+            + q * (120.3125 + q * (-44.0 + 6.5625 * q))))));
+    }
+}
+
+__device__ __forceinline__ double self_coeff(int nk)
+{
+    return nk == 0 ? 1.4 : (nk == 1 ? 1.9140625 : 1.71875);
+}
+
+// One side k of a softened pair: adds F(r,H_k) and P(r,H_k) (without m_j).
+__device__ __forceinline__ void soft_side(int nk, double r, double ri, double r2, double H,
+                                          double invH, double &gacc, double &gpot)
+{
+    if (r2 <= H * H) {
+        double A, P;
+        poly_AP(nk, r * invH, H * ri, A, P);
+        gacc += A * invH * invH * invH;
+        gpot += P * invH;
+    } else {
+        gacc += ri * ri * ri;
+        gpot -= ri;
+    }
+}
+
+// Ordered pair (i <- j), FP64, the semantics of grav_force_direct.cu:55-396
+// with the r == 0 rule documented in include/sphgrav.h.
+__device__ __forceinline__ void near_pair(int nk, double dx, double dy, double dz, double Hi,
+                                          double invHi, double Hj, double mj, bool gasj,
+                                          bool same, double &ax, double &ay, double &az,
+                                          double &ap)
+{
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double hj2 = Hj * Hj;
+    if (r2 >= fmax(Hi * Hi, hj2)) {
+        const double ri = rsqrt(r2);
+        const double mr = mj * ri;
+        const double mr3 = mr * ri * ri;
+        ax = fma(mr3, dx, ax);
+        ay = fma(mr3, dy, ay);
+        az = fma(mr3, dz, az);
+        ap -= mr;
+    } else if (r2 == 0.0) {
+        if (!same || gasj) ap -= self_coeff(nk) * mj * (invHi + 1.0 / Hj);
+    } else {
+        const double ri = rsqrt(r2);
+        const double r = r2 * ri;
+        double gacc = 0.0, gpot = 0.0;
+        soft_side(nk, r, ri, r2, Hi, invHi, gacc, gpot);
+        soft_side(nk, r, ri, r2, Hj, 1.0 / Hj, gacc, gpot);
+        const double hm = 0.5 * mj;
+        const double f = hm * gacc;
+        ax = fma(f, dx, ax);
+        ay = fma(f, dy, ay);
+        az = fma(f, dz, az);
+        ap = fma(hm, gpot, ap);
+    }
+}
+
+// Near kernel: one workgroup (one wave) per (i-group, j-partition); walks the
+// tiles its far-kernel wave flagged and adds their pairs, in FP64, to the
+// partial sums the far kernel left.  j is read straight from the sorted FP64
+// arrays with wave-uniform addresses.
+template <int IPL>
+__global__ __launch_bounds__(64) void k_near(PairArgs a)
+{
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x % a.ngroups;
+    const int jp = (int)(blockIdx.x / a.ngroups);
+    int t0, t1;
+    tile_range(a, jp, t0, t1);
+    const unsigned char *flags = a.nearflag + g * a.ntiles;
+
+    double xi[IPL], yi[IPL], zi[IPL], Hi[IPL], invHi[IPL];
+    double ax[IPL], ay[IPL], az[IPL], ap[IPL];
+    int idi[IPL];
+    int64_t ii[IPL];
+    bool valid[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        const int64_t i = g * (64 * IPL) + k * 64 + lane;
+        valid[k] = i < a.ni;
+        ii[k] = valid[k] ? i : a.ni - 1;
+        xi[k] = a.i.x[ii[k]];
+        yi[k] = a.i.y[ii[k]];
+        zi[k] = a.i.z[ii[k]];
+        Hi[k] = fabs(a.i.Hs[ii[k]]);
+        invHi[k] = 1.0 / Hi[k];
+        idi[k] = a.i.idx[ii[k]];
+        ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+    }
+    bool any = false;
+    for (int t = t0; t < t1; ++t) {
+        if (!flags[t]) continue;
+        any = true;
+        const int64_t j0 = (int64_t)t * TJ;
+        for (int jj = 0; jj < TJ; ++jj) {
+            const int64_t j = j0 + jj;
+            const double xj = a.j.x[j], yj = a.j.y[j], zj = a.j.z[j];
+            const double mj = a.j.m[j], Hs = a.j.Hs[j];
+            const int idj = a.j.idx[j];
+            const double Hj = fabs(Hs);
+#pragma unroll
+            for (int k = 0; k < IPL; ++k)
+                near_pair(a.nkernel, xj - xi[k], yj - yi[k], zj - zi[k], Hi[k], invHi[k], Hj, mj,
+                          Hs > 0.0, idj == idi[k], ax[k], ay[k], az[k], ap[k]);
+        }
+    }
+    if (!any) return;
+    double *p = partial_of(a, jp);
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        if (!valid[k]) continue;
+        p[ii[k]] += ax[k];
+        p[a.ni + ii[k]] += ay[k];
+        p[2 * a.ni + ii[k]] += az[k];
+        p[3 * a.ni + ii[k]] += ap[k];
+    }
+}
+
+// Fixed-order FP64 sum of the j-partition partials, scattered to input order.
+__global__ __launch_bounds__(256) void k_reduce_partials(PairArgs a)
+{
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i >= a.ni) return;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int jp = 0; jp < a.jsplit; ++jp) {
+        const double *p = a.partial + (int64_t)jp * 4 * a.ni;
+        for (int c = 0; c < 4; ++c) s[c] += p[c * a.ni + i];
+    }
+    const int64_t o = (int64_t)a.i.idx[i] - a.ibeg;
+    a.ax[o] = s[0];
+    a.ay[o] = s[1];
+    a.az[o] = s[2];
+    a.pot[o] = s[3];
+}
+
+inline dim3 grid_of(const PairArgs &a) { return dim3((unsigned)a.niblocks * (unsigned)a.jsplit); }
+
+}  // namespace
+
+#define SG_FAR(KERNEL, WG, IPL)                                                        \
+    if (wg == WG && a.ipl == IPL) {                                                    \
+        hipLaunchKernelGGL((KERNEL<WG, IPL>), grid_of(a), dim3(WG), 0, s, a);          \
+        return hipGetLastError();                                                      \
+    }
+
+hipError_t launch_pair_far(int precision, const PairArgs &a, int wg, hipStream_t s)
+{
+    if (a.ni <= 0) return hipSuccess;
+    if (precision == 0) {
+        SG_FAR(k_far_fp32c, 64, 1) SG_FAR(k_far_fp32c, 64, 2) SG_FAR(k_far_fp32c, 64, 4)
+        SG_FAR(k_far_fp32c, 128, 1) SG_FAR(k_far_fp32c, 128, 2) SG_FAR(k_far_fp32c, 128, 4)
+        SG_FAR(k_far_fp32c, 256, 1) SG_FAR(k_far_fp32c, 256, 2) SG_FAR(k_far_fp32c, 256, 4)
+    } else {
+        SG_FAR(k_far_fp64, 64, 1) SG_FAR(k_far_fp64, 64, 2)
+        SG_FAR(k_far_fp64, 128, 1) SG_FAR(k_far_fp64, 128, 2)
+        SG_FAR(k_far_fp64, 256, 1) SG_FAR(k_far_fp64, 256, 2)
+    }
+    return hipErrorInvalidConfiguration;
+}
+
+hipError_t launch_pair_near(const PairArgs &a, hipStream_t s)
+{
+    if (a.ni <= 0) return hipSuccess;
+    const dim3 grid((unsigned)(a.ngroups * a.jsplit));
+    switch (a.ipl) {
+    case 1: hipLaunchKernelGGL(k_near<1>, grid, dim3(64), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_near<2>, grid, dim3(64), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_near<4>, grid, dim3(64), 0, s, a); break;
+    default: return hipErrorInvalidConfiguration;
+    }
+    return hipGetLastError();
+}
+
+bool pair_config_supported(int precision, int wg, int ipl)
+{
+    const bool wg_ok = wg == 64 || wg == 128 || wg == 256;
+    if (precision == 0) return wg_ok && (ipl == 1 || ipl == 2 || ipl == 4);
+    return wg_ok && (ipl == 1 || ipl == 2);
+}
+
+hipError_t launch_reduce_partials(const PairArgs &a, hipStream_t s)
+{
+    if (a.ni <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((a.ni + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace sg

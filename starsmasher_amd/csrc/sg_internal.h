@@ -1,0 +1,100 @@
+// sg_internal.h -- shared declarations between the host layer (sg_capi.cpp)
+// and the device code (sg_direct.hip, sg_prep.hip, sg_sort.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sg {
+
+// Particles are visited in j-tiles of TJ consecutive (space-filling-curve
+// sorted) particles.  The sorted arrays are padded to a multiple of TJ with
+// massless copies of the last particle.
+constexpr int TJ = 256;
+
+// Per j-tile bounding box (FP64, absolute coordinates) and largest support
+// radius.  64 bytes.
+struct TileMeta {
+    double lo[3];
+    double hi[3];
+    double maxH;
+    double pad;
+};
+
+// Sorted, structure-of-arrays particle set as the pair kernels read it.
+//   x,y,z : FP64 positions
+//   m     : FP64 mass (0 for padding)
+//   Hs    : FP64 support radius H = sqrt|range2|, negative for point
+//           particles (range2 < 0, no self potential)
+//   idx   : original particle index (-1 for padding)
+struct SortedSet {
+    const double *x, *y, *z, *m, *Hs;
+    const int *idx;
+    int64_t count;      // real particles
+    int64_t padded;     // multiple of TJ
+};
+
+struct PairArgs {
+    SortedSet j;             // all n particles
+    SortedSet i;             // i-range particles in sorted order (may alias j)
+    const TileMeta *tiles;   // padded/TJ entries
+    int64_t ni;              // i-particles
+    int ibeg;                // outputs go to out[idx - ibeg]
+    int nkernel;
+    int ntiles;              // j-tiles
+    int jsplit;              // j partitions per i-block
+    int niblocks;            // i-blocks
+    int ipl;                 // i-particles per lane of the far kernel (groups of 64*ipl)
+    int64_t ngroups;         // i-groups (waves of the far kernel)
+    double *ax, *ay, *az, *pot;   // final outputs, input order, length ni
+    double *partial;              // [jsplit][4][ni] sorted-order partial sums
+    unsigned char *nearflag;      // [ngroups][ntiles]: 1 = tile takes the softened path
+    unsigned long long *stats;    // [2]: tile visits, near visits (may be null)
+};
+
+// Launchers (sg_direct.hip).  Return hipError_t of the launch.
+// An evaluation is far kernel -> near kernel -> reduce, in stream order.
+hipError_t launch_pair_far(int precision, const PairArgs &a, int wg, hipStream_t s);
+hipError_t launch_pair_near(const PairArgs &a, hipStream_t s);
+hipError_t launch_reduce_partials(const PairArgs &a, hipStream_t s);
+bool pair_config_supported(int precision, int wg, int ipl);
+
+// Preparation (sg_prep.hip).
+struct PrepBuffers {
+    // scratch
+    double *red;          // [nred_blocks * 8] reduction partials
+    int nred_blocks;
+    double *bbox;         // [8]: lo xyz, hi xyz, (unused)
+    int *err;             // [1]: validation bit flags
+    uint64_t *keys_in, *keys_out;
+    int *vals_in, *vals_out;
+    // sorted output arrays (capacity >= padded)
+    double *sx, *sy, *sz, *sm, *sH;
+    int *sidx;
+    TileMeta *tiles;
+    // i-subset sorted arrays
+    double *ix, *iy, *iz, *im, *iH;
+    int *iidx;
+};
+
+enum ErrBits { ERR_MASS = 1, ERR_RANGE2 = 2, ERR_NONFINITE = 4 };
+
+hipError_t launch_validate_bbox(const double *x, const double *y, const double *z,
+                                const double *m, const double *r2, int n,
+                                PrepBuffers &b, hipStream_t s);
+hipError_t launch_morton(const double *x, const double *y, const double *z, int first, int count,
+                         const PrepBuffers &b, hipStream_t s);
+hipError_t launch_gather(const double *x, const double *y, const double *z,
+                         const double *m, const double *r2, const int *perm, int count,
+                         int64_t padded, double *ox, double *oy, double *oz, double *om,
+                         double *oH, int *oidx, hipStream_t s);
+hipError_t launch_tile_meta(const double *sx, const double *sy, const double *sz,
+                            const double *sH, int ntiles, TileMeta *tiles, hipStream_t s);
+hipError_t launch_fill_zero_int(int *p, int n, hipStream_t s);
+
+// Sorting (sg_sort.hip): radix sort of 63-bit keys with int payloads.
+hipError_t sort_pairs_temp_bytes(int n, size_t *bytes);
+hipError_t sort_pairs(void *tmp, size_t tmp_bytes, const uint64_t *kin, uint64_t *kout,
+                      const int *vin, int *vout, int n, hipStream_t s);
+
+}  // namespace sg

@@ -1,0 +1,238 @@
+// sg_prep.hip -- O(N) preparation of the particle set for the pair kernels.
+//
+// The reference packs FP64 host arrays into a 32-byte FP32 AoS record per
+// particle on the host (grav_force_direct.cu:588-612).  Here the set is
+// prepared on the device from FP64 device arrays:
+//   1. validate (mass > 0, range2 != 0, finite; :601-602) and bounding box;
+//   2. 63-bit Morton keys (21 bits per axis) of the positions;
+//   3. radix sort of (key, index) -- sg_sort.hip;
+//   4. gather into sorted FP64 structure-of-arrays, H = sqrt|range2| signed
+//      by the gas/point flag (:603-605), padded to a multiple of TJ with
+//      massless copies of the last particle;
+//   5. per-tile bounding box and largest H (near/far test of the pair kernel).
+#include "sg_internal.h"
+
+#include <math.h>
+
+namespace sg {
+namespace {
+
+constexpr int RB = 256;   // threads per block for the O(N) kernels
+
+__device__ inline double warp_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double warp_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide min/max of 6 doubles (lo xyz, hi xyz); result valid in thread 0.
+__device__ inline void block_minmax6(double v[6])
+{
+    __shared__ double sh[RB / 64][6];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int c = 0; c < 3; ++c) v[c] = warp_min(v[c]);
+    for (int c = 3; c < 6; ++c) v[c] = warp_max(v[c]);
+    if (lane == 0)
+        for (int c = 0; c < 6; ++c) sh[w][c] = v[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int ww = 1; ww < RB / 64; ++ww) {
+            for (int c = 0; c < 3; ++c) v[c] = fmin(v[c], sh[ww][c]);
+            for (int c = 3; c < 6; ++c) v[c] = fmax(v[c], sh[ww][c]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(RB) void k_fill_zero_int(int *p, int n)
+{
+    const int t = blockIdx.x * RB + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+__global__ __launch_bounds__(RB) void k_validate_bbox(const double *__restrict__ x,
+                                                      const double *__restrict__ y,
+                                                      const double *__restrict__ z,
+                                                      const double *__restrict__ m,
+                                                      const double *__restrict__ r2, int n,
+                                                      double *__restrict__ red, int *err)
+{
+    double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    int flags = 0;
+    for (int64_t k = blockIdx.x * (int64_t)RB + threadIdx.x; k < n; k += (int64_t)gridDim.x * RB) {
+        const double px = x[k], py = y[k], pz = z[k], pm = m[k], pr = r2[k];
+        if (!(isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(pm) && isfinite(pr)))
+            flags |= ERR_NONFINITE;
+        if (!(pm > 0.0)) flags |= ERR_MASS;
+        if (pr == 0.0) flags |= ERR_RANGE2;
+        v[0] = fmin(v[0], px); v[1] = fmin(v[1], py); v[2] = fmin(v[2], pz);
+        v[3] = fmax(v[3], px); v[4] = fmax(v[4], py); v[5] = fmax(v[5], pz);
+    }
+    if (flags) atomicOr(err, flags);
+    block_minmax6(v);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 6; ++c) red[blockIdx.x * 8 + c] = v[c];
+}
+
+__global__ __launch_bounds__(RB) void k_bbox_final(const double *__restrict__ red, int nblocks,
+                                                   double *__restrict__ bbox)
+{
+    double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nblocks; b += RB) {
+        for (int c = 0; c < 3; ++c) v[c] = fmin(v[c], red[b * 8 + c]);
+        for (int c = 3; c < 6; ++c) v[c] = fmax(v[c], red[b * 8 + c]);
+    }
+    block_minmax6(v);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 6; ++c) bbox[c] = v[c];
+}
+
+// Spread the low 21 bits of v so that bit b lands at bit 3b (bit interleave
+// by successive shift-and-mask; public-domain bit-twiddling construction).
+__device__ inline uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__device__ inline uint64_t quant21(double v, double lo, double scale)
+{
+    double t = (v - lo) * scale;
+    t = fmin(fmax(t, 0.0), 2097151.0);
+    return (uint64_t)t;
+}
+
+__global__ __launch_bounds__(RB) void k_morton(const double *__restrict__ x,
+                                               const double *__restrict__ y,
+                                               const double *__restrict__ z, int first, int count,
+                                               const double *__restrict__ bbox,
+                                               uint64_t *__restrict__ keys, int *__restrict__ vals)
+{
+    const int k = blockIdx.x * RB + threadIdx.x;
+    if (k >= count) return;
+    const int p = first + k;
+    // one cube for all axes keeps the curve isotropic
+    const double ext = fmax(fmax(bbox[3] - bbox[0], bbox[4] - bbox[1]), bbox[5] - bbox[2]);
+    const double scale = ext > 0.0 ? 2097151.0 / ext : 0.0;
+    const uint64_t kx = quant21(x[p], bbox[0], scale);
+    const uint64_t ky = quant21(y[p], bbox[1], scale);
+    const uint64_t kz = quant21(z[p], bbox[2], scale);
+    keys[k] = (spread3(kx) << 2) | (spread3(ky) << 1) | spread3(kz);
+    vals[k] = p;
+}
+
+__global__ __launch_bounds__(RB) void k_gather(const double *__restrict__ x,
+                                               const double *__restrict__ y,
+                                               const double *__restrict__ z,
+                                               const double *__restrict__ m,
+                                               const double *__restrict__ r2,
+                                               const int *__restrict__ perm, int count,
+                                               int64_t padded, double *__restrict__ ox,
+                                               double *__restrict__ oy, double *__restrict__ oz,
+                                               double *__restrict__ om, double *__restrict__ oH,
+                                               int *__restrict__ oidx)
+{
+    const int64_t k = blockIdx.x * (int64_t)RB + threadIdx.x;
+    if (k >= padded) return;
+    const bool real = k < count;
+    const int src = perm[real ? k : count - 1];
+    const double rr = r2[src];
+    const double H = sqrt(rr > 0.0 ? rr : -rr);
+    ox[k] = x[src];
+    oy[k] = y[src];
+    oz[k] = z[src];
+    om[k] = real ? m[src] : 0.0;
+    oH[k] = (real && rr > 0.0) ? H : -H;   // padding carries no self potential
+    oidx[k] = real ? src : -1;
+}
+
+__global__ __launch_bounds__(TJ) void k_tile_meta(const double *__restrict__ sx,
+                                                  const double *__restrict__ sy,
+                                                  const double *__restrict__ sz,
+                                                  const double *__restrict__ sH,
+                                                  TileMeta *__restrict__ tiles)
+{
+    static_assert(TJ == RB, "one thread per tile particle");
+    const int64_t k = (int64_t)blockIdx.x * TJ + threadIdx.x;
+    double v[6] = {sx[k], sy[k], sz[k], sx[k], sy[k], sz[k]};
+    double h = fabs(sH[k]);
+    block_minmax6(v);
+    __shared__ double shH[TJ / 64];
+    h = warp_max(h);
+    if ((threadIdx.x & 63) == 0) shH[threadIdx.x >> 6] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < TJ / 64; ++w) h = fmax(h, shH[w]);
+        TileMeta t;
+        for (int c = 0; c < 3; ++c) {
+            t.lo[c] = v[c];
+            t.hi[c] = v[3 + c];
+        }
+        t.maxH = h;
+        t.pad = 0.0;
+        tiles[blockIdx.x] = t;
+    }
+}
+
+inline int nblocks_for(int64_t n) { return (int)((n + RB - 1) / RB); }
+
+}  // namespace
+
+hipError_t launch_fill_zero_int(int *p, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_zero_int, dim3(nblocks_for(n)), dim3(RB), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_validate_bbox(const double *x, const double *y, const double *z,
+                                const double *m, const double *r2, int n, PrepBuffers &b,
+                                hipStream_t s)
+{
+    hipError_t e = launch_fill_zero_int(b.err, 1, s);
+    if (e != hipSuccess) return e;
+    const int nb = std::min(b.nred_blocks, std::max(1, nblocks_for(n)));
+    hipLaunchKernelGGL(k_validate_bbox, dim3(nb), dim3(RB), 0, s, x, y, z, m, r2, n, b.red, b.err);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(RB), 0, s, b.red, nb, b.bbox);
+    return hipGetLastError();
+}
+
+hipError_t launch_morton(const double *x, const double *y, const double *z, int first, int count,
+                         const PrepBuffers &b, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_morton, dim3(nblocks_for(count)), dim3(RB), 0, s, x, y, z, first, count,
+                       b.bbox, b.keys_in, b.vals_in);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const double *x, const double *y, const double *z, const double *m,
+                         const double *r2, const int *perm, int count, int64_t padded,
+                         double *ox, double *oy, double *oz, double *om, double *oH, int *oidx,
+                         hipStream_t s)
+{
+    if (padded <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3(nblocks_for(padded)), dim3(RB), 0, s, x, y, z, m, r2, perm,
+                       count, padded, ox, oy, oz, om, oH, oidx);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_meta(const double *sx, const double *sy, const double *sz,
+                            const double *sH, int ntiles, TileMeta *tiles, hipStream_t s)
+{
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_meta, dim3(ntiles), dim3(TJ), 0, s, sx, sy, sz, sH, tiles);
+    return hipGetLastError();
+}
+
+}  // namespace sg

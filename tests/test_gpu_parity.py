@@ -1,0 +1,234 @@
+"""GPU parity of the HIP gravity library against the FP64 oracle.
+
+Tolerances (relative, per particle; |a - a_ref| / |a_ref| and |phi - phi_ref| / |phi_ref|):
+  FP64 kernel : acceleration <= 1e-11, potential <= 1e-12 (summation order only)
+  FP32C kernel: acceleration max <= 1e-5 (the path's accuracy target) and
+                median <= 1e-6; potential max <= 1e-6
+Golden potentials stored by the reference in its dumps: <= 4e-7 (their own
+FP32 floor), both precisions.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from starsmasher_amd import SG_PREC_FP32C, SG_PREC_FP64, Gravity, ics, _lib
+from starsmasher_amd.gravity import firsthalf_grav_forces, lasthalf_grav_forces
+
+from conftest import rel_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL = {SG_PREC_FP64: (1e-11, 1e-11, 1e-12), SG_PREC_FP32C: (1e-5, 1e-6, 1e-6)}
+PNAME = {SG_PREC_FP32C: "fp32c", SG_PREC_FP64: "fp64"}
+
+
+@pytest.fixture(scope="module")
+def grav():
+    g = Gravity(0)
+    yield g
+    g.close()
+
+
+def _check(tag, prec, res, ref):
+    ax, ay, az, pot = res
+    rx, ry, rz, rp = ref
+    ea, ep = rel_errors(np.stack([ax, ay, az], -1), pot, np.stack([rx, ry, rz], -1), rp)
+    amax, amed, pmax = TOL[prec]
+    print(f"[{tag} {PNAME[prec]}] acc max {ea.max():.3e} med {np.median(ea):.3e} "
+          f"pot max {ep.max():.3e} med {np.median(ep):.3e}", flush=True)
+    assert np.all(np.isfinite(ea)) and np.all(np.isfinite(ep))
+    assert ea.max() <= amax, (tag, ea.max())
+    assert np.median(ea) <= amed, (tag, np.median(ea))
+    assert ep.max() <= pmax, (tag, ep.max())
+
+
+def _args(d):
+    return d["x"], d["y"], d["z"], d["m"], d["range2"]
+
+
+@pytest.fixture(scope="module")
+def plummer16k():
+    d = ics.plummer(16384, seed=1)
+    ref = oracle.direct(*_args(d), nkernel=0)
+    return d, ref
+
+
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+def test_plummer_16k(grav, plummer16k, prec):
+    d, ref = plummer16k
+    _check("plummer16k", prec, grav.compute(*_args(d), nkernel=0, precision=prec), ref)
+
+
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+def test_golden_dump_potentials(grav, golden_dumps, prec):
+    for name, d in golden_dumps.items():
+        args = (d["x"], d["y"], d["z"], d["am"], d["range2"])
+        ax, ay, az, pot = grav.compute(*args, nkernel=0, precision=prec)
+        rel = np.abs(pot - d["grpot"]) / np.abs(d["grpot"])
+        print(f"[dump {name} {PNAME[prec]}] grpot max {rel.max():.3e} med {np.median(rel):.3e}")
+        assert rel.max() < 4e-7
+        _check(f"dump {name}", prec, (ax, ay, az, pot), oracle.direct(*args, nkernel=0))
+
+
+@pytest.mark.parametrize("nk", [1, 2])
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+def test_wendland_kernels_with_point_particles(grav, nk, prec):
+    d = ics.plummer(6000, seed=7)
+    d["range2"][-5:] *= -1.0          # five point particles
+    _check(f"nkernel{nk}", prec, grav.compute(*_args(d), nkernel=nk, precision=prec),
+           oracle.direct(*_args(d), nkernel=nk))
+
+
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+@pytest.mark.parametrize("ibeg,ni", [(0, 1), (1234, 1700), (0, 4999), (4999, 1), (2500, 2500),
+                                     (17, 0)])
+def test_i_ranges(grav, prec, ibeg, ni):
+    d = ics.plummer(5000, seed=8)
+    res = grav.compute(*_args(d), nkernel=0, precision=prec, ibeg=ibeg, ni=ni)
+    assert all(r.size == ni for r in res)
+    if ni == 0:
+        return
+    ref = oracle.direct(*_args(d), nkernel=0, rows=np.arange(ibeg, ibeg + ni))
+    _check(f"irange{ibeg}+{ni}", prec, res, ref)
+
+
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 255, 256, 257, 1000])
+def test_small_and_ragged_sizes(grav, prec, n):
+    r = np.random.default_rng(n)
+    pos = r.normal(size=(n, 3))
+    m = r.uniform(0.5, 1.5, n) / n
+    hp = r.uniform(0.2, 0.8, n)
+    u = np.ones(n)
+    u[0] = 0.0 if n > 1 else 1.0
+    r2 = ics.range2_of(hp, u)
+    res = grav.compute(pos[:, 0], pos[:, 1], pos[:, 2], m, r2, nkernel=0, precision=prec)
+    ref = oracle.direct(pos[:, 0], pos[:, 1], pos[:, 2], m, r2, nkernel=0)
+    if n == 1:
+        assert res[0][0] == 0.0 and abs(res[3][0] - ref[3][0]) <= 1e-12 * abs(ref[3][0])
+        return
+    _check(f"n{n}", prec, res, ref)
+
+
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+def test_coincident_particles_are_finite(grav, prec):
+    """A distinct pair at r = 0 returns the finite r->0 limit (the reference
+    kernel returns NaN there, grav_force_direct.cu:110-112)."""
+    d = ics.plummer(3000, seed=9)
+    for k in (10, 11, 12):
+        d["x"][k], d["y"][k], d["z"][k] = d["x"][5], d["y"][5], d["z"][5]
+    res = grav.compute(*_args(d), nkernel=0, precision=prec)
+    assert all(np.all(np.isfinite(r)) for r in res)
+    _check("coincident", prec, res, oracle.direct(*_args(d), nkernel=0))
+
+
+def test_legacy_fortran_abi(grav):
+    """The reference's three symbols, by reference, through ctypes: results
+    land in [n_lower-1, n_lower-1+my_length) and nothing else is touched."""
+    L = _lib.load()
+    d = ics.plummer(4000, seed=11)
+    n, n_lower, my_len = 4000, 801, 1500
+    ip = lambda v: C.byref(C.c_int(v))
+    dp = C.POINTER(C.c_double)
+    L.gpu_init_dev_(ip(0), C.byref(C.c_double(0.3)))
+    arrs = [np.ascontiguousarray(d[k]) for k in ("x", "y", "z", "m", "range2")]
+    os.environ["SPHGRAV_QUIET"] = "1"
+    L.firsthalf_grav_forces_(ip(n), ip(n_lower), ip(my_len), *[a.ctypes.data_as(dp) for a in arrs],
+                             ip(1), ip(0))
+    assert L.sg_last_status() == 0
+    outs = [np.full(n, 12345.0) for _ in range(4)]
+    L.lasthalf_grav_forces_(ip(n), *[o.ctypes.data_as(dp) for o in outs], ip(0))
+    assert L.sg_last_status() == 0
+    lo, hi = n_lower - 1, n_lower - 1 + my_len
+    for o in outs:
+        assert np.all(o[:lo] == 12345.0) and np.all(o[hi:] == 12345.0)
+    ref = oracle.direct(*arrs, nkernel=0, rows=np.arange(lo, hi))
+    _check("legacy-abi", SG_PREC_FP32C, [o[lo:hi] for o in outs], ref)
+
+
+def test_python_mirror_of_reference_interface():
+    d = ics.plummer(2000, seed=12)
+    firsthalf_grav_forces(2000, 1, 2000, d["x"], d["y"], d["z"], d["m"], d["range2"], 1, 0)
+    outs = [np.zeros(2000) for _ in range(4)]
+    lasthalf_grav_forces(2000, *outs)
+    _check("mirror", SG_PREC_FP32C, outs, oracle.direct(*_args(d), nkernel=0))
+
+
+def test_device_api_matches_host_api(grav):
+    torch = pytest.importorskip("torch")
+    d = ics.plummer(7000, seed=13)
+    dev = [torch.from_numpy(d[k]).to("cuda:0") for k in ("x", "y", "z", "m", "range2")]
+    out = [torch.empty(7000, dtype=torch.float64, device="cuda:0") for _ in range(4)]
+    grav.compute_device(7000, *dev, *out, nkernel=0, precision=SG_PREC_FP32C)
+    grav.synchronize()
+    host = grav.compute(*_args(d), nkernel=0, precision=SG_PREC_FP32C)
+    for a, b in zip(out, host):
+        np.testing.assert_array_equal(a.cpu().numpy(), b)   # same kernels, same order
+
+
+def test_device_validation_errors(grav):
+    torch = pytest.importorskip("torch")
+    d = ics.plummer(1000, seed=14)
+    for key, val, status in (("m", 0.0, _lib.SG_ERR_MASS), ("range2", 0.0, _lib.SG_ERR_RANGE2),
+                             ("x", np.inf, _lib.SG_ERR_NONFINITE)):
+        arrs = {k: d[k].copy() for k in ("x", "y", "z", "m", "range2")}
+        arrs[key][37] = val
+        dev = [torch.from_numpy(arrs[k]).to("cuda:0") for k in ("x", "y", "z", "m", "range2")]
+        out = [torch.empty(1000, dtype=torch.float64, device="cuda:0") for _ in range(4)]
+        grav.compute_device(1000, *dev, *out)
+        with pytest.raises(_lib.SphgravError) as e:
+            grav.synchronize()
+        assert e.value.status == status
+
+
+def test_deterministic(grav, plummer16k):
+    d, _ = plummer16k
+    a = grav.compute(*_args(d), nkernel=0)
+    b = grav.compute(*_args(d), nkernel=0)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("prec,wg,ipl", [(SG_PREC_FP32C, w, i) for w in (64, 128, 256) for i in (1, 2, 4)]
+                         + [(SG_PREC_FP64, w, i) for w in (64, 128, 256) for i in (1, 2)])
+@pytest.mark.parametrize("jsplit", [0, 1, 3])
+def test_tile_configurations(grav, plummer16k, prec, wg, ipl, jsplit):
+    d, ref = plummer16k
+    old = grav.get_config(prec)
+    grav.set_config(prec, wg, ipl, jsplit)
+    try:
+        _check(f"cfg{wg}x{ipl}/{jsplit}", prec, grav.compute(*_args(d), nkernel=0, precision=prec), ref)
+    finally:
+        grav.set_config(prec, *old)
+
+
+@pytest.fixture(scope="module")
+def two_poly():
+    return ics.two_polytropes()
+
+
+def test_two_polytropes_262k(grav, two_poly):
+    """Full size: FP32C against the FP64 kernel on every particle, against the
+    oracle on sampled rows (including the worst FP32C rows), and momentum
+    conservation of both."""
+    d = two_poly
+    r64 = grav.compute(*_args(d), nkernel=0, precision=SG_PREC_FP64)
+    r32 = grav.compute(*_args(d), nkernel=0, precision=SG_PREC_FP32C)
+    ea, ep = rel_errors(np.stack(r32[:3], -1), r32[3], np.stack(r64[:3], -1), r64[3])
+    print(f"[twopoly fp32c vs fp64 kernel] acc max {ea.max():.3e} med {np.median(ea):.3e} "
+          f"pot max {ep.max():.3e}")
+    rng = np.random.default_rng(0)
+    rows = np.unique(np.concatenate([rng.choice(d["x"].size, 384, replace=False),
+                                     np.argsort(ea)[-64:], np.argsort(ep)[-64:]]))
+    ref = oracle.direct(*_args(d), nkernel=0, rows=rows)
+    _check("twopoly rows", SG_PREC_FP64, [r[rows] for r in r64], ref)
+    _check("twopoly rows", SG_PREC_FP32C, [r[rows] for r in r32], ref)
+    assert ea.max() <= 1e-5 and ep.max() <= 1e-6
+    m = d["m"]
+    for r in (r64, r32):
+        a = np.stack(r[:3], -1)
+        tot = np.linalg.norm((m[:, None] * a).sum(0))
+        assert tot <= 1e-7 * (m * np.linalg.norm(a, axis=1)).sum()
