@@ -51,12 +51,12 @@ def make_workload(name):
 
 
 def time_precision(g, dev, n, prec, steps, warmup):
-    import torch
-    out = [torch.empty(n, dtype=torch.float64, device="cuda:0") for _ in range(4)]
+    from starsmasher_amd.hipmem import DeviceArray, runtime
+    out = [DeviceArray(n) for _ in range(4)]
     for _ in range(warmup):
         g.compute_device(n, *dev, *out, nkernel=0, precision=prec)
         g.synchronize()
-    torch.cuda.synchronize()
+    runtime().hipDeviceSynchronize()
     far_ms, near_ms, prep_ms = [], [], []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -66,10 +66,10 @@ def time_precision(g, dev, n, prec, steps, warmup):
         prep_ms.append(p)
         far_ms.append(f)
         near_ms.append(nr)
-    torch.cuda.synchronize()
+    runtime().hipDeviceSynchronize()
     t1 = time.perf_counter()
     visits, nearv = g.last_tile_stats()
-    res = [o.cpu().numpy() for o in out]
+    res = [o.download() for o in out]
     return {
         "ms_per_step": (t1 - t0) * 1e3 / steps,
         "far_ms": float(np.mean(far_ms)), "near_ms": float(np.mean(near_ms)),
@@ -123,15 +123,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
     from starsmasher_amd import SG_PREC_FP32C, SG_PREC_FP64, Gravity
+    from starsmasher_amd.hipmem import DeviceArray
 
     log(f"bench: generating workload {args.workload}")
     d, desc = make_workload(args.workload)
     n = d["x"].size
-    dev = [torch.from_numpy(np.ascontiguousarray(d[k])).to("cuda:0")
-           for k in ("x", "y", "z", "m", "range2")]
     g = Gravity(0)
+    # inputs resident in HBM before the timed region
+    dev = [DeviceArray.from_host(d[k]) for k in ("x", "y", "z", "m", "range2")]
     wg, ipl, js = g.get_config(SG_PREC_FP32C)
 
     log(f"bench: FP32C n={n} steps={args.steps} warmup={args.warmup}")

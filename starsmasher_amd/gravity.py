@@ -98,10 +98,15 @@ def gravforce(x, y, z, am, hp, u, ngrav_lower=1, ngrav_upper=None, q=1, nkernel=
 
 
 def _dev_ptr(t, name, n=None):
-    """Device pointer of a torch tensor (float64, contiguous, on a GPU) or a
-    raw integer address."""
+    """Device pointer of a :class:`starsmasher_amd.hipmem.DeviceArray`, a raw
+    integer address, or a torch tensor (float64, contiguous, on a GPU; only
+    valid when torch and the library share one HIP runtime)."""
     if isinstance(t, int):
         return t
+    if hasattr(t, "ptr") and hasattr(t, "size"):
+        if n is not None and t.size < n:
+            raise ValueError(f"{name} has {t.size} entries, needs {n}")
+        return int(t.ptr)
     try:
         import torch
     except ImportError as e:   # pragma: no cover
@@ -183,7 +188,7 @@ class Gravity:
         """Enqueue a device-resident evaluation (torch tensors or addresses)."""
         n = int(n)
         ni = n - ibeg if ni is None else int(ni)
-        if not isinstance(x, int):
+        if type(x).__module__.startswith("torch"):
             # run on the caller's current torch stream, ordered after the
             # work that produced the inputs
             import torch
