@@ -349,7 +349,7 @@ int sg_compute_device(sg_context *c, int n, int ibeg, int ni, const double *x, c
     HIP_TRY(hipEventRecord(c->ev_k0, s));
     HIP_TRY(sg::launch_pair_far(precision, a, wg, s));
     HIP_TRY(hipEventRecord(c->ev_kf, s));
-    HIP_TRY(sg::launch_pair_near(a, s));
+    HIP_TRY(sg::launch_pair_near(precision, a, s));
     HIP_TRY(sg::launch_reduce_partials(a, s));
     HIP_TRY(hipEventRecord(c->ev_k1, s));
     c->launched = true;

@@ -418,28 +418,96 @@ __device__ __forceinline__ void near_pair(int nk, double dx, double dy, double d
     }
 }
 
-// Near kernel: one workgroup (one wave) per (i-group, j-partition); walks the
-// tiles its far-kernel wave flagged and adds their pairs, in FP64, to the
-// partial sums the far kernel left.  j is read straight from the sorted FP64
-// arrays with wave-uniform addresses.
-template <int IPL>
-__global__ __launch_bounds__(64) void k_near(PairArgs a)
+// ---------------------------------------------------------- near kernels
+// One workgroup of NEAR_WAVES waves per (i-group, j-partition).  The waves
+// deal the group's flagged tiles round-robin among themselves, stage each
+// tile in their own LDS slice, and at the end combine their sums in a fixed
+// order (wave 0 + 1 + 2 + 3) before adding them to the far kernel's partials.
+constexpr int NEAR_WAVES = 4;
+
+__device__ __forceinline__ void wave_lds_sync()
 {
-    const int lane = threadIdx.x;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Fixed-order combination of the per-wave FP64 sums; adds them to the
+// partials of j-partition jp.
+template <int IPL>
+__device__ __forceinline__ void near_combine(const PairArgs &a, int jp, int w, int lane,
+                                             const int64_t (&ii)[IPL], const bool (&valid)[IPL],
+                                             double (&ax)[IPL], double (&ay)[IPL],
+                                             double (&az)[IPL], double (&ap)[IPL])
+{
+    __shared__ double s_red[NEAR_WAVES - 1][4][IPL][64];
+    if (w > 0) {
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            s_red[w - 1][0][k][lane] = ax[k];
+            s_red[w - 1][1][k][lane] = ay[k];
+            s_red[w - 1][2][k][lane] = az[k];
+            s_red[w - 1][3][k][lane] = ap[k];
+        }
+    }
+    __syncthreads();
+    if (w != 0) return;
+    double *p = partial_of(a, jp);
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        for (int v = 0; v < NEAR_WAVES - 1; ++v) {
+            ax[k] += s_red[v][0][k][lane];
+            ay[k] += s_red[v][1][k][lane];
+            az[k] += s_red[v][2][k][lane];
+            ap[k] += s_red[v][3][k][lane];
+        }
+        if (!valid[k]) continue;
+        p[ii[k]] += ax[k];
+        p[a.ni + ii[k]] += ay[k];
+        p[2 * a.ni + ii[k]] += az[k];
+        p[3 * a.ni + ii[k]] += ap[k];
+    }
+}
+
+__device__ __forceinline__ bool group_has_near(const unsigned char *flags, int t0, int t1)
+{
+    for (int t = t0; t < t1; ++t)
+        if (flags[t]) return true;
+    return false;
+}
+
+// FP32C near kernel.  Every pair first takes the FP32 Newtonian body with
+// hi+lo position differences; pairs with r < max(H_i, H_j) (the softened
+// ones, a few per i) are masked out of it and evaluated in FP64 from the FP64
+// positions instead.  At the boundary r = max(H) both forms agree, so the FP32
+// classification only decides which of two equal expressions is used.
+template <int IPL>
+__global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_fp32c(PairArgs a)
+{
+    __shared__ float4 s_hi[NEAR_WAVES][TJ];   // x_hi, y_hi, z_hi, m
+    __shared__ float4 s_lo[NEAR_WAVES][TJ];   // x_lo, y_lo, z_lo, H
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t g = blockIdx.x % a.ngroups;
     const int jp = (int)(blockIdx.x / a.ngroups);
     int t0, t1;
     tile_range(a, jp, t0, t1);
     const unsigned char *flags = a.nearflag + g * a.ntiles;
+    if (!group_has_near(flags, t0, t1)) return;
+
+    const int64_t gb = g * (64 * IPL);
+    const int64_t io = min(gb + 32 * IPL, a.ni - 1);
+    const double Ox = a.i.x[io], Oy = a.i.y[io], Oz = a.i.z[io];
 
     double xi[IPL], yi[IPL], zi[IPL], Hi[IPL], invHi[IPL];
+    float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL], hi2[IPL];
     double ax[IPL], ay[IPL], az[IPL], ap[IPL];
+    float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
     int idi[IPL];
     int64_t ii[IPL];
     bool valid[IPL];
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
-        const int64_t i = g * (64 * IPL) + k * 64 + lane;
+        const int64_t i = gb + k * 64 + lane;
         valid[k] = i < a.ni;
         ii[k] = valid[k] ? i : a.ni - 1;
         xi[k] = a.i.x[ii[k]];
@@ -448,35 +516,134 @@ __global__ __launch_bounds__(64) void k_near(PairArgs a)
         Hi[k] = fabs(a.i.Hs[ii[k]]);
         invHi[k] = 1.0 / Hi[k];
         idi[k] = a.i.idx[ii[k]];
+        const double rx = xi[k] - Ox, ry = yi[k] - Oy, rz = zi[k] - Oz;
+        xh[k] = (float)rx; yh[k] = (float)ry; zh[k] = (float)rz;
+        xl[k] = (float)(rx - (double)xh[k]);
+        yl[k] = (float)(ry - (double)yh[k]);
+        zl[k] = (float)(rz - (double)zh[k]);
+        const float hf = (float)Hi[k];
+        hi2[k] = hf * hf;
         ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+        fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
     }
-    bool any = false;
+
+    int cnt = 0;
     for (int t = t0; t < t1; ++t) {
         if (!flags[t]) continue;
-        any = true;
+        if ((cnt++ % NEAR_WAVES) != w) continue;
         const int64_t j0 = (int64_t)t * TJ;
-        for (int jj = 0; jj < TJ; ++jj) {
+        wave_lds_sync();
+        for (int q = 0; q < TJ / 64; ++q) {
+            const int jj = q * 64 + lane;
             const int64_t j = j0 + jj;
-            const double xj = a.j.x[j], yj = a.j.y[j], zj = a.j.z[j];
-            const double mj = a.j.m[j], Hs = a.j.Hs[j];
-            const int idj = a.j.idx[j];
-            const double Hj = fabs(Hs);
+            const double rx = a.j.x[j] - Ox, ry = a.j.y[j] - Oy, rz = a.j.z[j] - Oz;
+            const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+            s_hi[w][jj] = make_float4(hx, hy, hz, (float)a.j.m[j]);
+            s_lo[w][jj] = make_float4((float)(rx - (double)hx), (float)(ry - (double)hy),
+                                      (float)(rz - (double)hz), (float)fabs(a.j.Hs[j]));
+        }
+        wave_lds_sync();
+        for (int jj = 0; jj < TJ; ++jj) {
+            const float4 p = s_hi[w][jj];
+            const float4 q = s_lo[w][jj];
+            const float hj2 = q.w * q.w;
 #pragma unroll
-            for (int k = 0; k < IPL; ++k)
-                near_pair(a.nkernel, xj - xi[k], yj - yi[k], zj - zi[k], Hi[k], invHi[k], Hj, mj,
-                          Hs > 0.0, idj == idi[k], ax[k], ay[k], az[k], ap[k]);
+            for (int k = 0; k < IPL; ++k) {
+                const float dx = (p.x - xh[k]) + (q.x - xl[k]);
+                const float dy = (p.y - yh[k]) + (q.y - yl[k]);
+                const float dz = (p.z - zh[k]) + (q.z - zl[k]);
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const bool soft = r2 < fmaxf(hi2[k], hj2);
+                const float ri = __builtin_amdgcn_rsqf(soft ? 1.0f : r2);
+                const float mr = (soft ? 0.0f : p.w) * ri;
+                const float mr3 = mr * ri * ri;
+                fx[k] = fmaf(mr3, dx, fx[k]);
+                fy[k] = fmaf(mr3, dy, fy[k]);
+                fz[k] = fmaf(mr3, dz, fz[k]);
+                fp[k] += mr;
+                if (soft) {
+                    const int64_t j = j0 + jj;
+                    const double Hs = a.j.Hs[j];
+                    near_pair(a.nkernel, a.j.x[j] - xi[k], a.j.y[j] - yi[k], a.j.z[j] - zi[k],
+                              Hi[k], invHi[k], fabs(Hs), a.j.m[j], Hs > 0.0,
+                              a.j.idx[j] == idi[k], ax[k], ay[k], az[k], ap[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            ax[k] += (double)fx[k];
+            ay[k] += (double)fy[k];
+            az[k] += (double)fz[k];
+            ap[k] -= (double)fp[k];
+            fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
         }
     }
-    if (!any) return;
-    double *p = partial_of(a, jp);
+    near_combine<IPL>(a, jp, w, lane, ii, valid, ax, ay, az, ap);
+}
+
+// Plain FP64 near kernel: the same tile walk, every pair in FP64.
+template <int IPL>
+__global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_fp64(PairArgs a)
+{
+    __shared__ double4 s_p[NEAR_WAVES][TJ];   // x, y, z relative to the origin; m
+    __shared__ double2 s_h[NEAR_WAVES][TJ];   // signed H, original index (as double)
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t g = blockIdx.x % a.ngroups;
+    const int jp = (int)(blockIdx.x / a.ngroups);
+    int t0, t1;
+    tile_range(a, jp, t0, t1);
+    const unsigned char *flags = a.nearflag + g * a.ntiles;
+    if (!group_has_near(flags, t0, t1)) return;
+
+    const int64_t gb = g * (64 * IPL);
+    const int64_t io = min(gb + 32 * IPL, a.ni - 1);
+    const double Ox = a.i.x[io], Oy = a.i.y[io], Oz = a.i.z[io];
+
+    double xr[IPL], yr[IPL], zr[IPL], Hi[IPL], invHi[IPL];
+    double ax[IPL], ay[IPL], az[IPL], ap[IPL];
+    int idi[IPL];
+    int64_t ii[IPL];
+    bool valid[IPL];
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
-        if (!valid[k]) continue;
-        p[ii[k]] += ax[k];
-        p[a.ni + ii[k]] += ay[k];
-        p[2 * a.ni + ii[k]] += az[k];
-        p[3 * a.ni + ii[k]] += ap[k];
+        const int64_t i = gb + k * 64 + lane;
+        valid[k] = i < a.ni;
+        ii[k] = valid[k] ? i : a.ni - 1;
+        xr[k] = a.i.x[ii[k]] - Ox;
+        yr[k] = a.i.y[ii[k]] - Oy;
+        zr[k] = a.i.z[ii[k]] - Oz;
+        Hi[k] = fabs(a.i.Hs[ii[k]]);
+        invHi[k] = 1.0 / Hi[k];
+        idi[k] = a.i.idx[ii[k]];
+        ax[k] = ay[k] = az[k] = ap[k] = 0.0;
     }
+
+    int cnt = 0;
+    for (int t = t0; t < t1; ++t) {
+        if (!flags[t]) continue;
+        if ((cnt++ % NEAR_WAVES) != w) continue;
+        const int64_t j0 = (int64_t)t * TJ;
+        wave_lds_sync();
+        for (int q = 0; q < TJ / 64; ++q) {
+            const int jj = q * 64 + lane;
+            const int64_t j = j0 + jj;
+            s_p[w][jj] = make_double4(a.j.x[j] - Ox, a.j.y[j] - Oy, a.j.z[j] - Oz, a.j.m[j]);
+            s_h[w][jj] = make_double2(a.j.Hs[j], (double)a.j.idx[j]);
+        }
+        wave_lds_sync();
+        for (int jj = 0; jj < TJ; ++jj) {
+            const double4 p = s_p[w][jj];
+            const double2 h = s_h[w][jj];
+#pragma unroll
+            for (int k = 0; k < IPL; ++k)
+                near_pair(a.nkernel, p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k], invHi[k],
+                          fabs(h.x), p.w, h.x > 0.0, (int)h.y == idi[k], ax[k], ay[k], az[k],
+                          ap[k]);
+        }
+    }
+    near_combine<IPL>(a, jp, w, lane, ii, valid, ax, ay, az, ap);
 }
 
 // Fixed-order FP64 sum of the j-partition partials, scattered to input order.
@@ -521,15 +688,23 @@ hipError_t launch_pair_far(int precision, const PairArgs &a, int wg, hipStream_t
     return hipErrorInvalidConfiguration;
 }
 
-hipError_t launch_pair_near(const PairArgs &a, hipStream_t s)
+hipError_t launch_pair_near(int precision, const PairArgs &a, hipStream_t s)
 {
     if (a.ni <= 0) return hipSuccess;
-    const dim3 grid((unsigned)(a.ngroups * a.jsplit));
-    switch (a.ipl) {
-    case 1: hipLaunchKernelGGL(k_near<1>, grid, dim3(64), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_near<2>, grid, dim3(64), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(k_near<4>, grid, dim3(64), 0, s, a); break;
-    default: return hipErrorInvalidConfiguration;
+    const dim3 grid((unsigned)(a.ngroups * a.jsplit)), block(64 * NEAR_WAVES);
+    if (precision == 0) {
+        switch (a.ipl) {
+        case 1: hipLaunchKernelGGL(k_near_fp32c<1>, grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_near_fp32c<2>, grid, block, 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_near_fp32c<4>, grid, block, 0, s, a); break;
+        default: return hipErrorInvalidConfiguration;
+        }
+    } else {
+        switch (a.ipl) {
+        case 1: hipLaunchKernelGGL(k_near_fp64<1>, grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_near_fp64<2>, grid, block, 0, s, a); break;
+        default: return hipErrorInvalidConfiguration;
+        }
     }
     return hipGetLastError();
 }

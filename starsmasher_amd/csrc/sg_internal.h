@@ -55,7 +55,7 @@ struct PairArgs {
 // Launchers (sg_direct.hip).  Return hipError_t of the launch.
 // An evaluation is far kernel -> near kernel -> reduce, in stream order.
 hipError_t launch_pair_far(int precision, const PairArgs &a, int wg, hipStream_t s);
-hipError_t launch_pair_near(const PairArgs &a, hipStream_t s);
+hipError_t launch_pair_near(int precision, const PairArgs &a, hipStream_t s);
 hipError_t launch_reduce_partials(const PairArgs &a, hipStream_t s);
 bool pair_config_supported(int precision, int wg, int ipl);
 

@@ -204,6 +204,31 @@ def test_tile_configurations(grav, plummer16k, prec, wg, ipl, jsplit):
         grav.set_config(prec, *old)
 
 
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+def test_far_path_dominates_with_small_h(grav, prec):
+    """Small softening lengths make almost every tile Newtonian, so the far
+    kernel's FP32 body (not the FP64 near path) produces the result."""
+    d = ics.plummer(20000, seed=15)
+    d["range2"] = d["range2"] * 1e-4          # H / 100
+    res = grav.compute(*_args(d), nkernel=0, precision=prec)
+    visits, nearv = grav.last_tile_stats()
+    print(f"[smallH] near tiles {nearv}/{visits}")
+    assert nearv < 0.05 * visits
+    _check("smallH", prec, res, oracle.direct(*_args(d), nkernel=0))
+
+
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+def test_giant_with_point_mass_rows(grav, prec):
+    """Masses over 4 decades, H over 3 decades and a 30% point mass (no self
+    potential, softening H = 0.2), against the oracle on sampled rows."""
+    d = ics.giant_with_core(65536, seed=16)
+    res = grav.compute(*_args(d), nkernel=0, precision=prec)
+    rng = np.random.default_rng(1)
+    rows = np.unique(np.concatenate([rng.choice(65536, 1500, replace=False), [65535]]))
+    ref = oracle.direct(*_args(d), nkernel=0, rows=rows)
+    _check("giant64k rows", prec, [r[rows] for r in res], ref)
+
+
 @pytest.fixture(scope="module")
 def two_poly():
     return ics.two_polytropes()
