@@ -90,35 +90,52 @@ __device__ __forceinline__ double *partial_of(const PairArgs &a, int jp)
 }
 
 // ------------------------------------------------ far kernel, FP32 + comp.
+// One LDS stage of TS j-particles against the wave's i-particles: FP32 sums
+// over FOLD consecutive j, each folded into the FP64 accumulators.
+constexpr int FOLD = 32;
+static_assert(TS % FOLD == 0, "stage must be a whole number of folds");
+
 template <int IPL, bool WIDE>
 __device__ __forceinline__ void far_stage_fp32(const float4 *s_hi, const float4 *s_lo,
                                                const float (&xh)[IPL], const float (&yh)[IPL],
                                                const float (&zh)[IPL], const float (&xl)[IPL],
                                                const float (&yl)[IPL], const float (&zl)[IPL],
-                                               float (&fx)[IPL], float (&fy)[IPL],
-                                               float (&fz)[IPL], float (&fp)[IPL])
+                                               double (&ax)[IPL], double (&ay)[IPL],
+                                               double (&az)[IPL], double (&ap)[IPL])
 {
+    for (int j0 = 0; j0 < TS; j0 += FOLD) {
+        float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
 #pragma unroll 4
-    for (int jj = 0; jj < TS; ++jj) {
-        const float4 p = s_hi[jj];
-        float4 q;
-        if (WIDE) q = s_lo[jj];
+        for (int jj = j0; jj < j0 + FOLD; ++jj) {
+            const float4 p = s_hi[jj];
+            float4 q;
+            if (WIDE) q = s_lo[jj];
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                float dx = p.x - xh[k], dy = p.y - yh[k], dz = p.z - zh[k];
+                if (WIDE) {
+                    dx += q.x - xl[k];
+                    dy += q.y - yl[k];
+                    dz += q.z - zl[k];
+                }
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const float ri = __builtin_amdgcn_rsqf(r2);
+                const float mr = p.w * ri;
+                const float mr3 = mr * ri * ri;
+                fx[k] = fmaf(mr3, dx, fx[k]);
+                fy[k] = fmaf(mr3, dy, fy[k]);
+                fz[k] = fmaf(mr3, dz, fz[k]);
+                fp[k] += mr;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
-            float dx = p.x - xh[k], dy = p.y - yh[k], dz = p.z - zh[k];
-            if (WIDE) {
-                dx += q.x - xl[k];
-                dy += q.y - yl[k];
-                dz += q.z - zl[k];
-            }
-            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            const float ri = __builtin_amdgcn_rsqf(r2);
-            const float mr = p.w * ri;
-            const float mr3 = mr * ri * ri;
-            fx[k] = fmaf(mr3, dx, fx[k]);
-            fy[k] = fmaf(mr3, dy, fy[k]);
-            fz[k] = fmaf(mr3, dz, fz[k]);
-            fp[k] += mr;
+            ax[k] += (double)fx[k];
+            ay[k] += (double)fy[k];
+            az[k] += (double)fz[k];
+            ap[k] -= (double)fp[k];
         }
     }
 }
@@ -177,13 +194,9 @@ __global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
     // could cost more than ~1e-6 relative in dr
     const bool wide = uniform_i(ext > WIDE_RATIO * hmin) != 0;
 
-    float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
     double ax[IPL], ay[IPL], az[IPL], ap[IPL];
 #pragma unroll
-    for (int k = 0; k < IPL; ++k) {
-        fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
-        ax[k] = ay[k] = az[k] = ap[k] = 0.0;
-    }
+    for (int k = 0; k < IPL; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
 
     unsigned nearv = 0;
     unsigned char *flags = a.nearflag + group * a.ntiles;
@@ -205,17 +218,9 @@ __global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
             __syncthreads();
             if (far) {
                 if (wide)
-                    far_stage_fp32<IPL, true>(s_hi, s_lo, xh, yh, zh, xl, yl, zl, fx, fy, fz, fp);
+                    far_stage_fp32<IPL, true>(s_hi, s_lo, xh, yh, zh, xl, yl, zl, ax, ay, az, ap);
                 else
-                    far_stage_fp32<IPL, false>(s_hi, s_lo, xh, yh, zh, xl, yl, zl, fx, fy, fz, fp);
-#pragma unroll
-                for (int k = 0; k < IPL; ++k) {
-                    ax[k] += (double)fx[k];
-                    ay[k] += (double)fy[k];
-                    az[k] += (double)fz[k];
-                    ap[k] -= (double)fp[k];
-                    fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
-                }
+                    far_stage_fp32<IPL, false>(s_hi, s_lo, xh, yh, zh, xl, yl, zl, ax, ay, az, ap);
             }
         }
     }
@@ -501,7 +506,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_fp32c(PairArgs a)
     double xi[IPL], yi[IPL], zi[IPL], Hi[IPL], invHi[IPL];
     float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL], hi2[IPL];
     double ax[IPL], ay[IPL], az[IPL], ap[IPL];
-    float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
+
     int idi[IPL];
     int64_t ii[IPL];
     bool valid[IPL];
@@ -524,7 +529,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_fp32c(PairArgs a)
         const float hf = (float)Hi[k];
         hi2[k] = hf * hf;
         ax[k] = ay[k] = az[k] = ap[k] = 0.0;
-        fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
+
     }
 
     int cnt = 0;
@@ -557,10 +562,11 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_fp32c(PairArgs a)
                 const float ri = __builtin_amdgcn_rsqf(soft ? 1.0f : r2);
                 const float mr = (soft ? 0.0f : p.w) * ri;
                 const float mr3 = mr * ri * ri;
-                fx[k] = fmaf(mr3, dx, fx[k]);
-                fy[k] = fmaf(mr3, dy, fy[k]);
-                fz[k] = fmaf(mr3, dz, fz[k]);
-                fp[k] += mr;
+                // near tiles hold the largest terms: each is added in FP64
+                ax[k] += (double)(mr3 * dx);
+                ay[k] += (double)(mr3 * dy);
+                az[k] += (double)(mr3 * dz);
+                ap[k] -= (double)mr;
                 if (soft) {
                     const int64_t j = j0 + jj;
                     const double Hs = a.j.Hs[j];
@@ -569,14 +575,6 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_fp32c(PairArgs a)
                               a.j.idx[j] == idi[k], ax[k], ay[k], az[k], ap[k]);
                 }
             }
-        }
-#pragma unroll
-        for (int k = 0; k < IPL; ++k) {
-            ax[k] += (double)fx[k];
-            ay[k] += (double)fy[k];
-            az[k] += (double)fz[k];
-            ap[k] -= (double)fp[k];
-            fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
         }
     }
     near_combine<IPL>(a, jp, w, lane, ii, valid, ax, ay, az, ap);
