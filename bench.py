@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
+                    help="autotune the tile shape of both kernels before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -132,6 +134,16 @@ def main():
     g = Gravity(0)
     # inputs resident in HBM before the timed region
     dev = [DeviceArray.from_host(d[k]) for k in ("x", "y", "z", "m", "range2")]
+    tuning = {}
+    if args.tune:
+        from starsmasher_amd.autotune import tune
+        outs = [DeviceArray(n) for _ in range(4)]
+        for prec in (SG_PREC_FP32C,) + (() if args.no_fp64 else (SG_PREC_FP64,)):
+            tr = tune(g, n, dev, outs, prec, reps=1, log=log)
+            tuning["fp32c" if prec == SG_PREC_FP32C else "fp64"] = {
+                "best": list(tr.best), "table_ms": [[r[0], r[1], r[2], round(r[3], 2)]
+                                                     for r in tr.table]}
+        del outs
     wg, ipl, js = g.get_config(SG_PREC_FP32C)
 
     log(f"bench: FP32C n={n} steps={args.steps} warmup={args.warmup}")
@@ -197,6 +209,7 @@ def main():
         "near_tile_fraction": round(near_frac, 5),
         "accuracy": acc,
         "cpu_baseline": cpu,
+        "autotune": tuning or None,
     }
     if r64 is not None:
         v64 = pairs / (r64["ms_per_step"] * 1e-3) / 1e9

@@ -1,0 +1,60 @@
+"""Tile-shape autotuner for the pair kernels.
+
+For a precision and a particle set already resident on the device, times
+every supported (workgroup size, i-particles per lane, j-partitions) shape of
+the far kernel and keeps the fastest whole evaluation (prep + far + near +
+reduce, HIP events).  The chosen shape is installed on the context with
+``sg_set_config`` and returned with the full timing table.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+from ._lib import SG_PREC_FP32C, SG_PREC_FP64
+
+CANDIDATES = {
+    SG_PREC_FP32C: [(64, 1), (64, 2), (64, 4), (128, 1), (128, 2), (256, 1), (256, 2), (256, 4)],
+    SG_PREC_FP64: [(64, 1), (64, 2), (128, 1), (128, 2), (256, 1), (256, 2)],
+}
+
+
+@dataclass
+class TuneResult:
+    precision: int
+    best: tuple
+    best_ms: float
+    table: list = field(default_factory=list)   # (wg, ipl, jsplit, total_ms, far_ms, near_ms)
+
+
+def tune(g, n, inputs, outputs, precision=SG_PREC_FP32C, nkernel=0, candidates=None,
+         jsplits=(0,), reps=2, log=None) -> TuneResult:
+    """Time each candidate shape on the given device-resident set.
+
+    g: a :class:`starsmasher_amd.Gravity`; inputs: 5 device arrays
+    (x, y, z, m, range2); outputs: 4 device arrays of length n.
+    """
+    cands = candidates or CANDIDATES[precision]
+    table = []
+    best, best_ms = None, float("inf")
+    for wg, ipl in cands:
+        for js in jsplits:
+            g.set_config(precision, wg, ipl, js)
+            g.compute_device(n, *inputs, *outputs, nkernel=nkernel, precision=precision)
+            g.synchronize()            # warm-up (buffers, code objects)
+            tot = far = near = 0.0
+            for _ in range(reps):
+                g.compute_device(n, *inputs, *outputs, nkernel=nkernel, precision=precision)
+                g.synchronize()
+                p, f, nr = g.last_timings()
+                tot += p + f + nr
+                far += f
+                near += nr
+            row = (wg, ipl, js, tot / reps, far / reps, near / reps)
+            table.append(row)
+            if log:
+                log(f"autotune prec={precision} wg={wg} ipl={ipl} jsplit={js}: "
+                    f"{row[3]:.2f} ms (far {row[4]:.2f}, near {row[5]:.2f})")
+            if row[3] < best_ms:
+                best, best_ms = (wg, ipl, js), row[3]
+    g.set_config(precision, *best)
+    return TuneResult(precision, best, best_ms, table)

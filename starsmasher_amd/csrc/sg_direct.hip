@@ -39,7 +39,10 @@ namespace {
 
 constexpr int TS = 128;                 // j per LDS stage; TJ / TS stages per tile
 constexpr float FAR_MARGIN = 1.0e-3f;   // relative guard on the box-distance test
-constexpr float WIDE_RATIO = 8.0f;      // wave extent / min H above which dr uses hi+lo
+// Hi-only differences x_j,hi - x_i,hi carry a relative error of about
+// 2^-24 (|x_i - O| + |x_j - O| + r) / r.  Tiles closer than WIDE_K times the
+// wave's reach from O use hi+lo differences, which bounds it by ~1.5 * 2^-24.
+constexpr float WIDE_K = 4.0f;
 
 static_assert(TJ % TS == 0, "tile must be a whole number of stages");
 
@@ -61,18 +64,23 @@ __device__ __forceinline__ float wave_max(float v)
     return uniform_f(v);
 }
 
-// Box-distance test of a wave's i-box [bl, bh] against tile tm, both
-// relative to the FP64 origin O.  True when every pair is at least the
-// larger support radius apart, i.e. every pair takes the Newtonian branch.
-__device__ __forceinline__ bool tile_is_far(const TileMeta &tm, double Ox, double Oy, double Oz,
-                                            const float (&bl)[3], const float (&bh)[3], float hmax)
+// Squared distance between a wave's i-box [bl, bh] and tile tm's box, both
+// relative to the FP64 origin O.
+__device__ __forceinline__ float box_dist2(const TileMeta &tm, double Ox, double Oy, double Oz,
+                                           const float (&bl)[3], const float (&bh)[3])
 {
     const float lx = (float)(tm.lo[0] - Ox), ly = (float)(tm.lo[1] - Oy), lz = (float)(tm.lo[2] - Oz);
     const float ux = (float)(tm.hi[0] - Ox), uy = (float)(tm.hi[1] - Oy), uz = (float)(tm.hi[2] - Oz);
     const float gx = fmaxf(0.0f, fmaxf(lx - bh[0], bl[0] - ux));
     const float gy = fmaxf(0.0f, fmaxf(ly - bh[1], bl[1] - uy));
     const float gz = fmaxf(0.0f, fmaxf(lz - bh[2], bl[2] - uz));
-    const float d2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+    return fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+}
+
+// True when every pair of the wave and the tile is at least the larger
+// support radius apart, i.e. every pair takes the Newtonian branch.
+__device__ __forceinline__ bool boxes_far(float d2, float hmax, const TileMeta &tm)
+{
     const float hm = fmaxf(hmax, (float)tm.maxH);
     return d2 > hm * hm * (1.0f + FAR_MARGIN);
 }
@@ -164,7 +172,7 @@ __global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
     int64_t ii[IPL];
     bool valid[IPL];
     float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
-    float hmax = 0.0f, hmin = INFINITY;
+    float hmax = 0.0f;
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
         const int64_t i = gbase + wv * WI + k * 64 + lane;
@@ -181,18 +189,15 @@ __global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
         bl[0] = fminf(bl[0], xh[k]); bl[1] = fminf(bl[1], yh[k]); bl[2] = fminf(bl[2], zh[k]);
         bh[0] = fmaxf(bh[0], xh[k]); bh[1] = fmaxf(bh[1], yh[k]); bh[2] = fmaxf(bh[2], zh[k]);
         hmax = fmaxf(hmax, H);
-        hmin = fminf(hmin, H);
     }
+    float reach = 0.0f;   // max |x_i - O| over the wave, per axis
     for (int c = 0; c < 3; ++c) {
         bl[c] = wave_min(bl[c]);
         bh[c] = wave_max(bh[c]);
+        reach = fmaxf(reach, fmaxf(-bl[c], bh[c]));
     }
     hmax = wave_max(hmax);
-    hmin = wave_min(hmin);
-    const float ext = fmaxf(fmaxf(bh[0] - bl[0], bh[1] - bl[1]), bh[2] - bl[2]);
-    // |x - O| of i and j reaches ext + r: take hi+lo differences when that
-    // could cost more than ~1e-6 relative in dr
-    const bool wide = uniform_i(ext > WIDE_RATIO * hmin) != 0;
+    const float wide_d2 = 3.0f * (WIDE_K * reach) * (WIDE_K * reach);
 
     double ax[IPL], ay[IPL], az[IPL], ap[IPL];
 #pragma unroll
@@ -201,7 +206,12 @@ __global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
     unsigned nearv = 0;
     unsigned char *flags = a.nearflag + group * a.ntiles;
     for (int t = t0; t < t1; ++t) {
-        const bool far = uniform_i(tile_is_far(a.tiles[t], Ox, Oy, Oz, bl, bh, hmax)) != 0;
+        const TileMeta &tm = a.tiles[t];
+        const float d2 = box_dist2(tm, Ox, Oy, Oz, bl, bh);
+        const bool far = uniform_i(boxes_far(d2, hmax, tm)) != 0;
+        // close tiles take hi+lo differences (reach is per axis, so the
+        // Euclidean reach is at most sqrt(3) times it)
+        const bool wide = uniform_i(d2 < wide_d2) != 0;
         if (lane == 0) flags[t] = far ? 0 : 1;
         nearv += far ? 0 : 1;
         if (WG == 64 && !far) continue;
@@ -292,7 +302,8 @@ __global__ __launch_bounds__(WG) void k_far_fp64(PairArgs a)
     unsigned nearv = 0;
     unsigned char *flags = a.nearflag + group * a.ntiles;
     for (int t = t0; t < t1; ++t) {
-        const bool far = uniform_i(tile_is_far(a.tiles[t], Ox, Oy, Oz, bl, bh, hmax)) != 0;
+        const TileMeta &tm = a.tiles[t];
+        const bool far = uniform_i(boxes_far(box_dist2(tm, Ox, Oy, Oz, bl, bh), hmax, tm)) != 0;
         if (lane == 0) flags[t] = far ? 0 : 1;
         nearv += far ? 0 : 1;
         if (WG == 64 && !far) continue;
@@ -389,8 +400,21 @@ __device__ __forceinline__ void soft_side(int nk, double r, double ri, double r2
     }
 }
 
+// 1/sqrt(x) in FP64: FAST takes the FP32 hardware estimate (1 ulp) and one
+// Newton step, y' = y + y (1 - x y^2) / 2, leaving ~1e-14 relative error;
+// otherwise the FP64 library rsqrt.
+template <bool FAST>
+__device__ __forceinline__ double rsqrt64(double x)
+{
+    if (!FAST || !(x > 1.0e-36 && x < 1.0e36)) return rsqrt(x);   // outside FP32 range
+    const double y = (double)__builtin_amdgcn_rsqf((float)x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
 // Ordered pair (i <- j), FP64, the semantics of grav_force_direct.cu:55-396
 // with the r == 0 rule documented in include/sphgrav.h.
+template <bool FAST>
 __device__ __forceinline__ void near_pair(int nk, double dx, double dy, double dz, double Hi,
                                           double invHi, double Hj, double mj, bool gasj,
                                           bool same, double &ax, double &ay, double &az,
@@ -399,7 +423,7 @@ __device__ __forceinline__ void near_pair(int nk, double dx, double dy, double d
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const double hj2 = Hj * Hj;
     if (r2 >= fmax(Hi * Hi, hj2)) {
-        const double ri = rsqrt(r2);
+        const double ri = rsqrt64<FAST>(r2);
         const double mr = mj * ri;
         const double mr3 = mr * ri * ri;
         ax = fma(mr3, dx, ax);
@@ -409,7 +433,7 @@ __device__ __forceinline__ void near_pair(int nk, double dx, double dy, double d
     } else if (r2 == 0.0) {
         if (!same || gasj) ap -= self_coeff(nk) * mj * (invHi + 1.0 / Hj);
     } else {
-        const double ri = rsqrt(r2);
+        const double ri = rsqrt64<FAST>(r2);
         const double r = r2 * ri;
         double gacc = 0.0, gpot = 0.0;
         soft_side(nk, r, ri, r2, Hi, invHi, gacc, gpot);
@@ -480,109 +504,12 @@ __device__ __forceinline__ bool group_has_near(const unsigned char *flags, int t
     return false;
 }
 
-// FP32C near kernel.  Every pair first takes the FP32 Newtonian body with
-// hi+lo position differences; pairs with r < max(H_i, H_j) (the softened
-// ones, a few per i) are masked out of it and evaluated in FP64 from the FP64
-// positions instead.  At the boundary r = max(H) both forms agree, so the FP32
-// classification only decides which of two equal expressions is used.
-template <int IPL>
-__global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_fp32c(PairArgs a)
-{
-    __shared__ float4 s_hi[NEAR_WAVES][TJ];   // x_hi, y_hi, z_hi, m
-    __shared__ float4 s_lo[NEAR_WAVES][TJ];   // x_lo, y_lo, z_lo, H
-
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int64_t g = blockIdx.x % a.ngroups;
-    const int jp = (int)(blockIdx.x / a.ngroups);
-    int t0, t1;
-    tile_range(a, jp, t0, t1);
-    const unsigned char *flags = a.nearflag + g * a.ntiles;
-    if (!group_has_near(flags, t0, t1)) return;
-
-    const int64_t gb = g * (64 * IPL);
-    const int64_t io = min(gb + 32 * IPL, a.ni - 1);
-    const double Ox = a.i.x[io], Oy = a.i.y[io], Oz = a.i.z[io];
-
-    double xi[IPL], yi[IPL], zi[IPL], Hi[IPL], invHi[IPL];
-    float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL], hi2[IPL];
-    double ax[IPL], ay[IPL], az[IPL], ap[IPL];
-
-    int idi[IPL];
-    int64_t ii[IPL];
-    bool valid[IPL];
-#pragma unroll
-    for (int k = 0; k < IPL; ++k) {
-        const int64_t i = gb + k * 64 + lane;
-        valid[k] = i < a.ni;
-        ii[k] = valid[k] ? i : a.ni - 1;
-        xi[k] = a.i.x[ii[k]];
-        yi[k] = a.i.y[ii[k]];
-        zi[k] = a.i.z[ii[k]];
-        Hi[k] = fabs(a.i.Hs[ii[k]]);
-        invHi[k] = 1.0 / Hi[k];
-        idi[k] = a.i.idx[ii[k]];
-        const double rx = xi[k] - Ox, ry = yi[k] - Oy, rz = zi[k] - Oz;
-        xh[k] = (float)rx; yh[k] = (float)ry; zh[k] = (float)rz;
-        xl[k] = (float)(rx - (double)xh[k]);
-        yl[k] = (float)(ry - (double)yh[k]);
-        zl[k] = (float)(rz - (double)zh[k]);
-        const float hf = (float)Hi[k];
-        hi2[k] = hf * hf;
-        ax[k] = ay[k] = az[k] = ap[k] = 0.0;
-
-    }
-
-    int cnt = 0;
-    for (int t = t0; t < t1; ++t) {
-        if (!flags[t]) continue;
-        if ((cnt++ % NEAR_WAVES) != w) continue;
-        const int64_t j0 = (int64_t)t * TJ;
-        wave_lds_sync();
-        for (int q = 0; q < TJ / 64; ++q) {
-            const int jj = q * 64 + lane;
-            const int64_t j = j0 + jj;
-            const double rx = a.j.x[j] - Ox, ry = a.j.y[j] - Oy, rz = a.j.z[j] - Oz;
-            const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
-            s_hi[w][jj] = make_float4(hx, hy, hz, (float)a.j.m[j]);
-            s_lo[w][jj] = make_float4((float)(rx - (double)hx), (float)(ry - (double)hy),
-                                      (float)(rz - (double)hz), (float)fabs(a.j.Hs[j]));
-        }
-        wave_lds_sync();
-        for (int jj = 0; jj < TJ; ++jj) {
-            const float4 p = s_hi[w][jj];
-            const float4 q = s_lo[w][jj];
-            const float hj2 = q.w * q.w;
-#pragma unroll
-            for (int k = 0; k < IPL; ++k) {
-                const float dx = (p.x - xh[k]) + (q.x - xl[k]);
-                const float dy = (p.y - yh[k]) + (q.y - yl[k]);
-                const float dz = (p.z - zh[k]) + (q.z - zl[k]);
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const bool soft = r2 < fmaxf(hi2[k], hj2);
-                const float ri = __builtin_amdgcn_rsqf(soft ? 1.0f : r2);
-                const float mr = (soft ? 0.0f : p.w) * ri;
-                const float mr3 = mr * ri * ri;
-                // near tiles hold the largest terms: each is added in FP64
-                ax[k] += (double)(mr3 * dx);
-                ay[k] += (double)(mr3 * dy);
-                az[k] += (double)(mr3 * dz);
-                ap[k] -= (double)mr;
-                if (soft) {
-                    const int64_t j = j0 + jj;
-                    const double Hs = a.j.Hs[j];
-                    near_pair(a.nkernel, a.j.x[j] - xi[k], a.j.y[j] - yi[k], a.j.z[j] - zi[k],
-                              Hi[k], invHi[k], fabs(Hs), a.j.m[j], Hs > 0.0,
-                              a.j.idx[j] == idi[k], ax[k], ay[k], az[k], ap[k]);
-                }
-            }
-        }
-    }
-    near_combine<IPL>(a, jp, w, lane, ii, valid, ax, ay, az, ap);
-}
-
-// Plain FP64 near kernel: the same tile walk, every pair in FP64.
-template <int IPL>
-__global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_fp64(PairArgs a)
+// Near kernel: every pair of the flagged tiles in FP64 from FP64 positions
+// relative to the group's origin.  FAST (the FP32C product) seeds 1/sqrt(r^2)
+// with the FP32 hardware estimate plus one FP64 Newton step; the plain FP64
+// kernel uses the FP64 library rsqrt.
+template <int IPL, bool FAST>
+__global__ __launch_bounds__(64 * NEAR_WAVES) void k_near(PairArgs a)
 {
     __shared__ double4 s_p[NEAR_WAVES][TJ];   // x, y, z relative to the origin; m
     __shared__ double2 s_h[NEAR_WAVES][TJ];   // signed H, original index (as double)
@@ -636,7 +563,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_fp64(PairArgs a)
             const double2 h = s_h[w][jj];
 #pragma unroll
             for (int k = 0; k < IPL; ++k)
-                near_pair(a.nkernel, p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k], invHi[k],
+                near_pair<FAST>(a.nkernel, p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k], invHi[k],
                           fabs(h.x), p.w, h.x > 0.0, (int)h.y == idi[k], ax[k], ay[k], az[k],
                           ap[k]);
         }
@@ -692,15 +619,15 @@ hipError_t launch_pair_near(int precision, const PairArgs &a, hipStream_t s)
     const dim3 grid((unsigned)(a.ngroups * a.jsplit)), block(64 * NEAR_WAVES);
     if (precision == 0) {
         switch (a.ipl) {
-        case 1: hipLaunchKernelGGL(k_near_fp32c<1>, grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_near_fp32c<2>, grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL(k_near_fp32c<4>, grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_near<1, true>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_near<2, true>), grid, block, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_near<4, true>), grid, block, 0, s, a); break;
         default: return hipErrorInvalidConfiguration;
         }
     } else {
         switch (a.ipl) {
-        case 1: hipLaunchKernelGGL(k_near_fp64<1>, grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_near_fp64<2>, grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_near<1, false>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_near<2, false>), grid, block, 0, s, a); break;
         default: return hipErrorInvalidConfiguration;
         }
     }
