@@ -118,6 +118,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tune", action=argparse.BooleanOptionalAction, default=True,
                     help="autotune the tile shape of both kernels before timing")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="first run one device-to-device copy of BYTES (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against a known transfer)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -134,6 +137,12 @@ def main():
     g = Gravity(0)
     # inputs resident in HBM before the timed region
     dev = [DeviceArray.from_host(d[k]) for k in ("x", "y", "z", "m", "range2")]
+    if args.copy_probe > 0:
+        src = DeviceArray(args.copy_probe // 8)
+        dst = DeviceArray(args.copy_probe // 8)
+        dst.copy_from(src)
+        log(f"bench: copy probe of {args.copy_probe} bytes done")
+        del src, dst
     tuning = {}
     if args.tune:
         from starsmasher_amd.autotune import tune

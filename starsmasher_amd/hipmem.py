@@ -18,6 +18,7 @@ _hip = None
 
 H2D = 1
 D2H = 2
+D2D = 3
 
 
 def runtime() -> C.CDLL:
@@ -81,6 +82,12 @@ class DeviceArray:
         _ok(runtime().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes, D2H),
             "hipMemcpy D2H")
         return out
+
+    def copy_from(self, other: "DeviceArray") -> None:
+        """Device-to-device copy (the runtime's plain copy kernel)."""
+        n = min(self.nbytes, other.nbytes)
+        _ok(runtime().hipMemcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr), n, D2D),
+            "hipMemcpy D2D")
 
     def free(self) -> None:
         if getattr(self, "ptr", 0):
