@@ -48,8 +48,9 @@ struct sg_context {
     double *in[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double *out[4] = {nullptr, nullptr, nullptr, nullptr};
 
-    int wg[2] = {64, 256};
-    int ipl[2] = {2, 1};
+    // tile shape per precision (autotuned on the N = 2^20 giant-star set)
+    int wg[2] = {256, 256};
+    int ipl[2] = {1, 1};
     int jsplit[2] = {0, 0};
 
     bool launched = false;

@@ -40,9 +40,10 @@ namespace {
 constexpr int TS = 128;                 // j per LDS stage; TJ / TS stages per tile
 constexpr float FAR_MARGIN = 1.0e-3f;   // relative guard on the box-distance test
 // Hi-only differences x_j,hi - x_i,hi carry a relative error of about
-// 2^-24 (|x_i - O| + |x_j - O| + r) / r.  Tiles closer than WIDE_K times the
-// wave's reach from O use hi+lo differences, which bounds it by ~1.5 * 2^-24.
-constexpr float WIDE_K = 4.0f;
+// 2^-24 (|x_i - O| + |x_j - O| + r) / r <= 2^-24 (2 reach + r) / r.  Tiles
+// closer than WIDE_K times the wave's reach from O use hi+lo differences,
+// which bounds it by 2^-24 (1 + 2 / WIDE_K) = 2 ulp elsewhere.
+constexpr float WIDE_K = 2.0f;
 
 static_assert(TJ % TS == 0, "tile must be a whole number of stages");
 
