@@ -137,6 +137,27 @@ int sg_compute_host(sg_context *ctx, int n, int ibeg, int ni,
 int sg_set_config(sg_context *ctx, int precision, int wg, int ipl, int jsplit);
 int sg_get_config(sg_context *ctx, int precision, int *wg, int *ipl, int *jsplit);
 
+/* Batch of independent particle sets (no reference counterpart: the
+ * reference evaluates one set per call).  Set s holds particles
+ * offsets[s] .. offsets[s+1]-1 of the device arrays (offsets: HOST array of
+ * nsets+1 entries, offsets[0] = 0, every set non-empty); particles interact
+ * only within their set.  Outputs (device, length offsets[nsets]) are in
+ * input order.  All sets run in one persistent launch whose workgroups take
+ * (set, i-block) items from a queue, longest sets first.  With use_graph the
+ * queue reset, far, near and scatter kernels are captured once in a HIP graph
+ * and replayed while the plan, pointers and precision stay the same; the
+ * preparation (validation, sort, gather) runs eagerly before the replay.
+ * Asynchronous like sg_compute_device; sg_synchronize reports validation. */
+int sg_compute_batch_device(sg_context *ctx, int nsets, const long long *offsets,
+                            const double *x, const double *y, const double *z,
+                            const double *m, const double *range2,
+                            int nkernel, int precision, int use_graph,
+                            double *ax, double *ay, double *az, double *pot);
+
+/* Tile shape of the batch kernel (wg 64/128/256, ipl 1/2) per precision. */
+int sg_set_batch_config(sg_context *ctx, int precision, int wg, int ipl);
+int sg_get_batch_config(sg_context *ctx, int precision, int *wg, int *ipl);
+
 /* HIP-event timings of the last completed evaluation, in milliseconds:
  * prep = validation, curve sort, gather and tile boxes; far = the far-field
  * pair kernel alone; near = near-field kernel plus the final reduction.

@@ -30,7 +30,8 @@ SG_PREC_FP64 = 1
 EXPORTS = (
     "firsthalf_grav_forces_", "lasthalf_grav_forces_", "gpu_init_dev_",
     "sg_status_string", "sg_last_status", "sg_device_count", "sg_create", "sg_destroy",
-    "sg_set_stream", "sg_compute_device", "sg_synchronize", "sg_last_kernel_ms", "sg_compute_host",
+    "sg_set_stream", "sg_compute_device", "sg_compute_batch_device", "sg_set_batch_config",
+    "sg_get_batch_config", "sg_synchronize", "sg_last_kernel_ms", "sg_compute_host",
     "sg_set_config", "sg_get_config", "sg_last_timings", "sg_last_tile_stats", "sg_init_device",
     "sg_firsthalf", "sg_lasthalf",
 )
@@ -76,6 +77,13 @@ def load() -> C.CDLL:
     L.sg_compute_host.argtypes = [_vp, C.c_int, C.c_int, C.c_int] + [_dp] * 5 + [C.c_int, C.c_int] + [_dp] * 4
     L.sg_synchronize.argtypes = [_vp]
     L.sg_set_stream.argtypes = [_vp, _vp]
+    L.sg_compute_batch_device.argtypes = ([_vp, C.c_int, C.POINTER(C.c_longlong)] + [_vp] * 5
+                                          + [C.c_int, C.c_int, C.c_int] + [_vp] * 4)
+    L.sg_compute_batch_device.restype = C.c_int
+    L.sg_set_batch_config.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.sg_set_batch_config.restype = C.c_int
+    L.sg_get_batch_config.argtypes = [_vp, C.c_int, _ip, _ip]
+    L.sg_get_batch_config.restype = C.c_int
     L.sg_set_stream.restype = C.c_int
     L.sg_last_kernel_ms.argtypes = [_vp, C.POINTER(C.c_float)]
     L.sg_set_config.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int]

@@ -54,6 +54,28 @@ struct sg_context {
     int jsplit[2] = {0, 0};
 
     bool launched = false;
+
+    // ---- batch of independent sets
+    int bwg[2] = {256, 256};
+    int bipl[2] = {1, 1};
+    struct Plan {
+        std::vector<int64_t> offsets;
+        int wg = 0, ipl = 0;
+        int64_t padded = 0, ngroups = 0, nflags = 0;
+        int nitems = 0;
+    } plan;
+    int64_t *d_offsets = nullptr;
+    int64_t cap_offsets = 0;
+    sg::BatchSet *d_sets = nullptr;
+    int64_t cap_sets = 0;
+    int2 *d_items = nullptr;
+    int64_t cap_items = 0;
+    unsigned *d_counter = nullptr;
+    // captured compute of the last batch call (replayed when nothing changed)
+    hipGraphExec_t gexec = nullptr;
+    sg::BatchArgs g_args{};
+    int g_key[5] = {-1, -1, -1, -1, -1};   // precision, nkernel, wg, ipl, nwg
+    hipStream_t g_stream = nullptr;
 };
 
 namespace {
@@ -89,10 +111,10 @@ int grow(T **p, int64_t *cap, int64_t need)
     return SG_OK;
 }
 
-// Capacity for a set of n particles (j) and ni i-particles.
-int ensure_capacity(sg_context *c, int64_t n, int64_t ni)
+// Capacity for n input particles (keys), `padded` sorted slots and ni
+// i-particles.
+int ensure_capacity(sg_context *c, int64_t n, int64_t padded, int64_t ni)
 {
-    const int64_t padded = ((n + TJ - 1) / TJ) * TJ;
     if (n > c->cap_n || !c->pb.keys_in) {
         int64_t cap = 0;
         const int64_t need = std::max<int64_t>(n, 1);
@@ -247,7 +269,9 @@ int sg_destroy(sg_context *c)
                     c->pb.sH, c->pb.sidx, c->pb.tiles, c->pb.ix, c->pb.iy, c->pb.iz,
                     c->pb.im, c->pb.iH, c->pb.iidx, c->sort_tmp, c->partial, c->nearflag, c->stats,
                     c->in[0], c->in[1], c->in[2], c->in[3], c->in[4],
-                    c->out[0], c->out[1], c->out[2], c->out[3]};
+                    c->out[0], c->out[1], c->out[2], c->out[3],
+                    c->d_offsets, c->d_sets, c->d_items, c->d_counter};
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     (void)hipEventDestroy(c->ev_p0);
@@ -294,7 +318,7 @@ int sg_compute_device(sg_context *c, int n, int ibeg, int ni, const double *x, c
     if (ni == 0) return SG_OK;
 
     const int64_t padded = ((int64_t)(n) + TJ - 1) / TJ * TJ;
-    int rc = ensure_capacity(c, n, ni);
+    int rc = ensure_capacity(c, n, padded, ni);
     if (rc != SG_OK) return rc;
     hipStream_t s = c->stream;
     sg::PrepBuffers &b = c->pb;
@@ -352,6 +376,169 @@ int sg_compute_device(sg_context *c, int n, int ibeg, int ni, const double *x, c
     HIP_TRY(hipEventRecord(c->ev_kf, s));
     HIP_TRY(sg::launch_pair_near(precision, a, s));
     HIP_TRY(sg::launch_reduce_partials(a, s));
+    HIP_TRY(hipEventRecord(c->ev_k1, s));
+    c->launched = true;
+    return SG_OK;
+}
+
+// ----------------------------------------------------- batch of sets
+int sg_set_batch_config(sg_context *c, int precision, int wg, int ipl)
+{
+    if (!c || (precision != SG_PREC_FP32C && precision != SG_PREC_FP64)) return SG_ERR_ARG;
+    if (!sg::batch_config_supported(wg, ipl)) return SG_ERR_CONFIG;
+    c->bwg[precision] = wg;
+    c->bipl[precision] = ipl;
+    return SG_OK;
+}
+
+int sg_get_batch_config(sg_context *c, int precision, int *wg, int *ipl)
+{
+    if (!c || (precision != SG_PREC_FP32C && precision != SG_PREC_FP64)) return SG_ERR_ARG;
+    if (wg) *wg = c->bwg[precision];
+    if (ipl) *ipl = c->bipl[precision];
+    return SG_OK;
+}
+
+namespace {
+
+// Host part of a batch plan: per-set layout, near-flag offsets and the work
+// items, longest sets first (their i-blocks cost the most tiles).
+int build_plan(sg_context *c, int nsets, const long long *offsets, int wg, int ipl)
+{
+    auto &P = c->plan;
+    P.offsets.assign(offsets, offsets + nsets + 1);
+    P.wg = wg;
+    P.ipl = ipl;
+    std::vector<sg::BatchSet> sets(nsets);
+    std::vector<int2> items;
+    const int64_t gi = (int64_t)wg * ipl;
+    int64_t pofs = 0, gofs = 0, fofs = 0;
+    for (int s = 0; s < nsets; ++s) {
+        const int64_t n = offsets[s + 1] - offsets[s];
+        const int64_t ntl = (n + TJ - 1) / TJ;
+        const int64_t nblk = (n + gi - 1) / gi;
+        sets[s] = sg::BatchSet{pofs, n, (int)(pofs / TJ), (int)ntl, gofs, fofs};
+        const int64_t groups = nblk * (wg / 64);   // every wave of every i-block
+        for (int64_t b = 0; b < nblk; ++b) items.push_back(make_int2(s, (int)b));
+        pofs += ntl * TJ;
+        gofs += groups;
+        fofs += groups * ntl;
+    }
+    std::stable_sort(items.begin(), items.end(), [&](const int2 &u, const int2 &v) {
+        return sets[u.x].ntiles > sets[v.x].ntiles;
+    });
+    P.padded = pofs;
+    P.ngroups = gofs;
+    P.nflags = fofs;
+    P.nitems = (int)items.size();
+    int rc;
+    if ((rc = grow(&c->d_offsets, &c->cap_offsets, nsets + 1)) != SG_OK) return rc;
+    if ((rc = grow(&c->d_sets, &c->cap_sets, nsets)) != SG_OK) return rc;
+    if ((rc = grow(&c->d_items, &c->cap_items, (int64_t)items.size())) != SG_OK) return rc;
+    if (!c->d_counter) HIP_TRY(hipMalloc((void **)&c->d_counter, 64));
+    HIP_TRY(hipMemcpy(c->d_offsets, P.offsets.data(), sizeof(int64_t) * (nsets + 1),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_sets, sets.data(), sizeof(sg::BatchSet) * nsets, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_items, items.data(), sizeof(int2) * items.size(),
+                      hipMemcpyHostToDevice));
+    return SG_OK;
+}
+
+}  // namespace
+
+int sg_compute_batch_device(sg_context *c, int nsets, const long long *offsets, const double *x,
+                            const double *y, const double *z, const double *m,
+                            const double *range2, int nkernel, int precision, int use_graph,
+                            double *ax, double *ay, double *az, double *pot)
+{
+    if (!c || nsets < 1 || !offsets) return SG_ERR_ARG;
+    if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
+    if (precision != SG_PREC_FP32C && precision != SG_PREC_FP64) return SG_ERR_ARG;
+    if (offsets[0] != 0 || nsets > (1 << 20)) return SG_ERR_ARG;
+    for (int s = 0; s < nsets; ++s)
+        if (offsets[s + 1] <= offsets[s]) return SG_ERR_ARG;   // every set holds >= 1 particle
+    const int64_t ntot = offsets[nsets];
+    if (ntot > 0x7fffffff) return SG_ERR_ARG;
+    if (!x || !y || !z || !m || !range2 || !ax || !ay || !az || !pot) return SG_ERR_ARG;
+    HIP_TRY(hipSetDevice(c->device));
+    c->launched = false;
+    const int wg = c->bwg[precision], ipl = c->bipl[precision];
+
+    auto &P = c->plan;
+    const bool same_plan = P.wg == wg && P.ipl == ipl && (int)P.offsets.size() == nsets + 1 &&
+                           std::equal(P.offsets.begin(), P.offsets.end(), offsets);
+    int rc;
+    if (!same_plan) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if ((rc = build_plan(c, nsets, offsets, wg, ipl)) != SG_OK) return rc;
+    }
+    if ((rc = ensure_capacity(c, ntot, P.padded, 1)) != SG_OK) return rc;
+    if ((rc = grow(&c->partial, &c->cap_part, 4 * P.padded)) != SG_OK) return rc;
+    if ((rc = grow(&c->nearflag, &c->cap_flag, std::max<int64_t>(P.nflags, 1))) != SG_OK) return rc;
+
+    hipStream_t s = c->stream;
+    sg::PrepBuffers &b = c->pb;
+    HIP_TRY(hipEventRecord(c->ev_p0, s));
+    // preparation (not captured: the radix sort clears its own scratch with
+    // memset nodes)
+    HIP_TRY(sg::launch_validate_bbox(x, y, z, m, range2, (int)ntot, b, s));
+    HIP_TRY(sg::launch_morton_batch(x, y, z, (int)ntot, c->d_offsets, nsets, b, s));
+    HIP_TRY(sg::sort_pairs(c->sort_tmp, c->sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
+                           b.vals_out, (int)ntot, s));
+    HIP_TRY(sg::launch_gather_batch(x, y, z, m, range2, b.vals_out, c->d_offsets, c->d_sets, nsets,
+                                    P.padded, b.sx, b.sy, b.sz, b.sm, b.sH, b.sidx, s));
+    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, (int)(P.padded / TJ), b.tiles, s));
+
+    sg::BatchArgs a{};
+    a.j = sg::SortedSet{b.sx, b.sy, b.sz, b.sm, b.sH, b.sidx, ntot, P.padded};
+    a.tiles = b.tiles;
+    a.sets = c->d_sets;
+    a.nsets = nsets;
+    a.items = c->d_items;
+    a.nitems = P.nitems;
+    a.ngroups = P.ngroups;
+    a.padded = P.padded;
+    a.nkernel = nkernel;
+    a.partial = c->partial;
+    a.nearflag = c->nearflag;
+    a.counter = c->d_counter;
+    a.ax = ax;
+    a.ay = ay;
+    a.az = az;
+    a.pot = pot;
+    a.stats = c->stats;
+    const int nwg = sg::batch_resident_workgroups(precision, wg, ipl, c->device);
+    if (nwg <= 0) return SG_ERR_CONFIG;
+
+    HIP_TRY(hipEventRecord(c->ev_k0, s));
+    if (use_graph) {
+        const int key[5] = {precision, nkernel, wg, ipl, nwg};
+        const bool reuse = c->gexec && c->g_stream == s &&
+                           std::memcmp(&c->g_args, &a, sizeof(a)) == 0 &&
+                           std::memcmp(c->g_key, key, sizeof(key)) == 0;
+        if (!reuse) {
+            if (c->gexec) HIP_TRY(hipGraphExecDestroy(c->gexec));
+            c->gexec = nullptr;
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 4, s));
+            const hipError_t le = sg::launch_batch(precision, a, wg, ipl, nwg, s);
+            const hipError_t ee = hipStreamEndCapture(s, &graph);
+            HIP_TRY(le);
+            HIP_TRY(ee);
+            const hipError_t ie = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIP_TRY(ie);
+            c->g_args = a;
+            std::memcpy(c->g_key, key, sizeof(key));
+            c->g_stream = s;
+        }
+        HIP_TRY(hipGraphLaunch(c->gexec, s));
+    } else {
+        HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 4, s));
+        HIP_TRY(sg::launch_batch(precision, a, wg, ipl, nwg, s));
+    }
+    HIP_TRY(hipEventRecord(c->ev_kf, s));
     HIP_TRY(hipEventRecord(c->ev_k1, s));
     c->launched = true;
     return SG_OK;

@@ -19,17 +19,21 @@
 //   * j data for far tiles is staged through LDS as FP32 {x_hi, y_hi, z_hi, m}
 //     and {x_lo, y_lo, z_lo} relative to an FP64 origin taken per workgroup,
 //     written from coalesced FP64 global loads and read back as same-address
-//     (broadcast) ds_read_b128.  Waves whose extent is large against their
-//     smallest H use hi+lo differences; compact waves use hi only;
-//   * FP32 partial sums are folded into FP64 accumulators after every LDS
-//     stage of TS j-particles (the cross-tile reduction is FP64);
-//   * NEAR kernel: the flagged (wave, tile) pairs, every pair through the full
-//     softened branch in FP64 (the Wendland polynomials carry coefficients up
-//     to 9009 that cancel to O(1) near q = 1, which FP32 cannot resolve);
+//     (broadcast) ds_read_b128.  Tiles close to the wave use hi+lo
+//     differences; distant ones hi only;
+//   * FP32 partial sums over FOLD j are folded into FP64 accumulators (the
+//     cross-tile reduction is FP64);
+//   * NEAR kernel: the flagged (wave, tile) pairs, every pair in FP64 (the
+//     Wendland polynomials carry coefficients up to 9009 that cancel to O(1)
+//     near q = 1, which FP32 cannot resolve);
 //   * REDUCE kernel: fixed-order FP64 sum over j-partitions, scattered back to
 //     input order.
 // The far kernel carries no FP64 softening code, so its register budget (and
 // occupancy) is set by the 14-instruction loop alone.
+//
+// Two launch forms share the block-level code: one particle set (grid over
+// i-blocks x j-partitions), and a batch of independent sets in one persistent
+// launch whose workgroups pull (set, i-block) items from a queue.
 #include "sg_internal.h"
 
 #include <math.h>
@@ -40,12 +44,18 @@ namespace {
 constexpr int TS = 128;                 // j per LDS stage; TJ / TS stages per tile
 constexpr float FAR_MARGIN = 1.0e-3f;   // relative guard on the box-distance test
 // Hi-only differences x_j,hi - x_i,hi carry a relative error of about
-// 2^-24 (|x_i - O| + |x_j - O| + r) / r <= 2^-24 (2 reach + r) / r.  Tiles
-// closer than WIDE_K times the wave's reach from O use hi+lo differences,
-// which bounds it by 2^-24 (1 + 2 / WIDE_K) = 2 ulp elsewhere.
-constexpr float WIDE_K = 2.0f;
+// 2^-24 (|x_i - O| + |x_j - O| + r) / r <= 2^-24 (2 + 2 |x_i - O| / r), with
+// |x_i - O| <= sqrt(3) reach (reach = per-axis max over the wave).  Tiles
+// nearer than sqrt(3) WIDE_K reach use hi+lo differences, which bounds the
+// hi-only ones by 2^-24 (2 + 2 / WIDE_K) = 6 ulp (3.6e-7).  Waves that
+// straddle a jump of the space-filling curve have a large reach; with
+// WIDE_K = 0.5 about 2% of tiles take the hi+lo body on the giant-star set.
+constexpr float WIDE_K = 0.5f;
+constexpr int FOLD = 32;                // FP32 partial sums span FOLD j
+constexpr int NEAR_WAVES = 4;           // waves sharing one group's near tiles
 
 static_assert(TJ % TS == 0, "tile must be a whole number of stages");
+static_assert(TS % FOLD == 0, "stage must be a whole number of folds");
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ float uniform_f(float v)
@@ -86,24 +96,27 @@ __device__ __forceinline__ bool boxes_far(float d2, float hmax, const TileMeta &
     return d2 > hm * hm * (1.0f + FAR_MARGIN);
 }
 
-__device__ __forceinline__ void tile_range(const PairArgs &a, int jp, int &t0, int &t1)
-{
-    t0 = (int)((int64_t)a.ntiles * jp / a.jsplit);
-    t1 = (int)((int64_t)a.ntiles * (jp + 1) / a.jsplit);
-}
+// The part of a launch one far-kernel workgroup works on.
+struct Block {
+    int64_t ibase;          // first (sorted) i of the workgroup
+    int64_t iend;           // end of the i segment the block lies in
+    int t0, t1;             // j-tile range
+    unsigned char *flags;   // near flags of the workgroup's first wave, [t - t0]
+    int64_t fstride;        // flag stride between consecutive waves
+    double *part;           // partial sums: part[c * pstride + i], c = ax, ay, az, pot
+    int64_t pstride;
+};
 
-// Sorted-order FP64 partial sums of one i for j-partition jp.
-__device__ __forceinline__ double *partial_of(const PairArgs &a, int jp)
-{
-    return a.partial + (int64_t)jp * 4 * a.ni;
-}
+// The part of a launch one near-kernel workgroup works on (one i-group).
+struct Group {
+    int64_t gbase, iend;
+    int t0, t1;
+    const unsigned char *flags;   // [t - t0]
+    double *part;
+    int64_t pstride;
+};
 
-// ------------------------------------------------ far kernel, FP32 + comp.
-// One LDS stage of TS j-particles against the wave's i-particles: FP32 sums
-// over FOLD consecutive j, each folded into the FP64 accumulators.
-constexpr int FOLD = 32;
-static_assert(TS % FOLD == 0, "stage must be a whole number of folds");
-
+// ------------------------------------------------ far block, FP32 + comp.
 template <int IPL, bool WIDE>
 __device__ __forceinline__ void far_stage_fp32(const float4 *s_hi, const float4 *s_lo,
                                                const float (&xh)[IPL], const float (&yh)[IPL],
@@ -150,24 +163,18 @@ __device__ __forceinline__ void far_stage_fp32(const float4 *s_hi, const float4 
 }
 
 template <int WG, int IPL>
-__global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
+__device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const SortedSet &J,
+                                                const TileMeta *tiles, const Block &b,
+                                                unsigned long long *stats, float4 *s_hi,
+                                                float4 *s_lo)
 {
     constexpr int WI = 64 * IPL;   // i per wave
     constexpr int GI = WG * IPL;   // i per workgroup
-    __shared__ float4 s_hi[TS];
-    __shared__ float4 s_lo[TS];
-
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int ib = blockIdx.x % a.niblocks;
-    const int jp = blockIdx.x / a.niblocks;
-    int t0, t1;
-    tile_range(a, jp, t0, t1);
-    const int64_t group = (int64_t)ib * (WG / 64) + wv;
 
     // FP64 origin of this workgroup: its middle i-particle.
-    const int64_t gbase = (int64_t)ib * GI;
-    const int64_t io = min(gbase + GI / 2, a.ni - 1);
-    const double Ox = a.i.x[io], Oy = a.i.y[io], Oz = a.i.z[io];
+    const int64_t io = min(b.ibase + GI / 2, b.iend - 1);
+    const double Ox = I.x[io], Oy = I.y[io], Oz = I.z[io];
 
     float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL];
     int64_t ii[IPL];
@@ -176,17 +183,17 @@ __global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
     float hmax = 0.0f;
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
-        const int64_t i = gbase + wv * WI + k * 64 + lane;
-        valid[k] = i < a.ni;
-        ii[k] = valid[k] ? i : a.ni - 1;
-        const double rx = a.i.x[ii[k]] - Ox, ry = a.i.y[ii[k]] - Oy, rz = a.i.z[ii[k]] - Oz;
+        const int64_t i = b.ibase + wv * WI + k * 64 + lane;
+        valid[k] = i < b.iend;
+        ii[k] = valid[k] ? i : b.iend - 1;
+        const double rx = I.x[ii[k]] - Ox, ry = I.y[ii[k]] - Oy, rz = I.z[ii[k]] - Oz;
         xh[k] = (float)rx;
         yh[k] = (float)ry;
         zh[k] = (float)rz;
         xl[k] = (float)(rx - (double)xh[k]);
         yl[k] = (float)(ry - (double)yh[k]);
         zl[k] = (float)(rz - (double)zh[k]);
-        const float H = (float)fabs(a.i.Hs[ii[k]]);
+        const float H = (float)fabs(I.Hs[ii[k]]);
         bl[0] = fminf(bl[0], xh[k]); bl[1] = fminf(bl[1], yh[k]); bl[2] = fminf(bl[2], zh[k]);
         bh[0] = fmaxf(bh[0], xh[k]); bh[1] = fmaxf(bh[1], yh[k]); bh[2] = fmaxf(bh[2], zh[k]);
         hmax = fmaxf(hmax, H);
@@ -198,36 +205,39 @@ __global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
         reach = fmaxf(reach, fmaxf(-bl[c], bh[c]));
     }
     hmax = wave_max(hmax);
+    // close tiles take hi+lo differences (reach is per axis, so the
+    // Euclidean reach is at most sqrt(3) times it)
     const float wide_d2 = 3.0f * (WIDE_K * reach) * (WIDE_K * reach);
 
     double ax[IPL], ay[IPL], az[IPL], ap[IPL];
 #pragma unroll
     for (int k = 0; k < IPL; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
 
+    // a wave past the end of its segment (the tail of the last i-block)
+    // computes nothing and flags nothing
+    const bool active = uniform_i(b.ibase + wv * WI < b.iend) != 0;
     unsigned nearv = 0;
-    unsigned char *flags = a.nearflag + group * a.ntiles;
-    for (int t = t0; t < t1; ++t) {
-        const TileMeta &tm = a.tiles[t];
+    unsigned char *flags = b.flags + wv * b.fstride;
+    for (int t = b.t0; t < b.t1; ++t) {
+        const TileMeta &tm = tiles[t];
         const float d2 = box_dist2(tm, Ox, Oy, Oz, bl, bh);
-        const bool far = uniform_i(boxes_far(d2, hmax, tm)) != 0;
-        // close tiles take hi+lo differences (reach is per axis, so the
-        // Euclidean reach is at most sqrt(3) times it)
+        const bool far = !active || uniform_i(boxes_far(d2, hmax, tm)) != 0;
         const bool wide = uniform_i(d2 < wide_d2) != 0;
-        if (lane == 0) flags[t] = far ? 0 : 1;
+        if (lane == 0) flags[t - b.t0] = far ? 0 : 1;
         nearv += far ? 0 : 1;
         if (WG == 64 && !far) continue;
         for (int s = 0; s < TJ / TS; ++s) {
             __syncthreads();
             for (int jj = tid; jj < TS; jj += WG) {
                 const int64_t j = (int64_t)t * TJ + s * TS + jj;
-                const double rx = a.j.x[j] - Ox, ry = a.j.y[j] - Oy, rz = a.j.z[j] - Oz;
+                const double rx = J.x[j] - Ox, ry = J.y[j] - Oy, rz = J.z[j] - Oz;
                 const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
-                s_hi[jj] = make_float4(hx, hy, hz, (float)a.j.m[j]);
+                s_hi[jj] = make_float4(hx, hy, hz, (float)J.m[j]);
                 s_lo[jj] = make_float4((float)(rx - (double)hx), (float)(ry - (double)hy),
                                        (float)(rz - (double)hz), 0.0f);
             }
             __syncthreads();
-            if (far) {
+            if (far && active) {
                 if (wide)
                     far_stage_fp32<IPL, true>(s_hi, s_lo, xh, yh, zh, xl, yl, zl, ax, ay, az, ap);
                 else
@@ -236,39 +246,32 @@ __global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
         }
     }
 
-    double *p = partial_of(a, jp);
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
         if (!valid[k]) continue;
-        p[ii[k]] = ax[k];
-        p[a.ni + ii[k]] = ay[k];
-        p[2 * a.ni + ii[k]] = az[k];
-        p[3 * a.ni + ii[k]] = ap[k];
+        b.part[ii[k]] = ax[k];
+        b.part[b.pstride + ii[k]] = ay[k];
+        b.part[2 * b.pstride + ii[k]] = az[k];
+        b.part[3 * b.pstride + ii[k]] = ap[k];
     }
-    if (a.stats && lane == 0) {
-        atomicAdd(&a.stats[0], (unsigned long long)(t1 - t0));
-        atomicAdd(&a.stats[1], (unsigned long long)nearv);
+    if (stats && lane == 0) {
+        atomicAdd(&stats[0], (unsigned long long)(b.t1 - b.t0));
+        atomicAdd(&stats[1], (unsigned long long)nearv);
     }
 }
 
-// ------------------------------------------------------- far kernel, FP64
+// ------------------------------------------------------- far block, FP64
 template <int WG, int IPL>
-__global__ __launch_bounds__(WG) void k_far_fp64(PairArgs a)
+__device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedSet &J,
+                                               const TileMeta *tiles, const Block &b,
+                                               unsigned long long *stats, double4 *s_p)
 {
     constexpr int WI = 64 * IPL;
     constexpr int GI = WG * IPL;
-    __shared__ double4 s_p[TS];
-
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int ib = blockIdx.x % a.niblocks;
-    const int jp = blockIdx.x / a.niblocks;
-    int t0, t1;
-    tile_range(a, jp, t0, t1);
-    const int64_t group = (int64_t)ib * (WG / 64) + wv;
 
-    const int64_t gbase = (int64_t)ib * GI;
-    const int64_t io = min(gbase + GI / 2, a.ni - 1);
-    const double Ox = a.i.x[io], Oy = a.i.y[io], Oz = a.i.z[io];
+    const int64_t io = min(b.ibase + GI / 2, b.iend - 1);
+    const double Ox = I.x[io], Oy = I.y[io], Oz = I.z[io];
 
     double xr[IPL], yr[IPL], zr[IPL];
     int64_t ii[IPL];
@@ -277,18 +280,18 @@ __global__ __launch_bounds__(WG) void k_far_fp64(PairArgs a)
     float hmax = 0.0f;
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
-        const int64_t i = gbase + wv * WI + k * 64 + lane;
-        valid[k] = i < a.ni;
-        ii[k] = valid[k] ? i : a.ni - 1;
-        xr[k] = a.i.x[ii[k]] - Ox;
-        yr[k] = a.i.y[ii[k]] - Oy;
-        zr[k] = a.i.z[ii[k]] - Oz;
+        const int64_t i = b.ibase + wv * WI + k * 64 + lane;
+        valid[k] = i < b.iend;
+        ii[k] = valid[k] ? i : b.iend - 1;
+        xr[k] = I.x[ii[k]] - Ox;
+        yr[k] = I.y[ii[k]] - Oy;
+        zr[k] = I.z[ii[k]] - Oz;
         // the same FP32 box as the FP32 kernel, so both classify tiles alike
         bl[0] = fminf(bl[0], (float)xr[k]); bl[1] = fminf(bl[1], (float)yr[k]);
         bl[2] = fminf(bl[2], (float)zr[k]);
         bh[0] = fmaxf(bh[0], (float)xr[k]); bh[1] = fmaxf(bh[1], (float)yr[k]);
         bh[2] = fmaxf(bh[2], (float)zr[k]);
-        hmax = fmaxf(hmax, (float)fabs(a.i.Hs[ii[k]]));
+        hmax = fmaxf(hmax, (float)fabs(I.Hs[ii[k]]));
     }
     for (int c = 0; c < 3; ++c) {
         bl[c] = wave_min(bl[c]);
@@ -300,22 +303,24 @@ __global__ __launch_bounds__(WG) void k_far_fp64(PairArgs a)
 #pragma unroll
     for (int k = 0; k < IPL; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
 
+    const bool active = uniform_i(b.ibase + wv * WI < b.iend) != 0;
     unsigned nearv = 0;
-    unsigned char *flags = a.nearflag + group * a.ntiles;
-    for (int t = t0; t < t1; ++t) {
-        const TileMeta &tm = a.tiles[t];
-        const bool far = uniform_i(boxes_far(box_dist2(tm, Ox, Oy, Oz, bl, bh), hmax, tm)) != 0;
-        if (lane == 0) flags[t] = far ? 0 : 1;
+    unsigned char *flags = b.flags + wv * b.fstride;
+    for (int t = b.t0; t < b.t1; ++t) {
+        const TileMeta &tm = tiles[t];
+        const bool far = !active ||
+                         uniform_i(boxes_far(box_dist2(tm, Ox, Oy, Oz, bl, bh), hmax, tm)) != 0;
+        if (lane == 0) flags[t - b.t0] = far ? 0 : 1;
         nearv += far ? 0 : 1;
         if (WG == 64 && !far) continue;
         for (int s = 0; s < TJ / TS; ++s) {
             __syncthreads();
             for (int jj = tid; jj < TS; jj += WG) {
                 const int64_t j = (int64_t)t * TJ + s * TS + jj;
-                s_p[jj] = make_double4(a.j.x[j] - Ox, a.j.y[j] - Oy, a.j.z[j] - Oz, a.j.m[j]);
+                s_p[jj] = make_double4(J.x[j] - Ox, J.y[j] - Oy, J.z[j] - Oz, J.m[j]);
             }
             __syncthreads();
-            if (!far) continue;
+            if (!far || !active) continue;
 #pragma unroll 2
             for (int jj = 0; jj < TS; ++jj) {
                 const double4 q = s_p[jj];
@@ -335,18 +340,17 @@ __global__ __launch_bounds__(WG) void k_far_fp64(PairArgs a)
         }
     }
 
-    double *p = partial_of(a, jp);
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
         if (!valid[k]) continue;
-        p[ii[k]] = ax[k];
-        p[a.ni + ii[k]] = ay[k];
-        p[2 * a.ni + ii[k]] = az[k];
-        p[3 * a.ni + ii[k]] = ap[k];
+        b.part[ii[k]] = ax[k];
+        b.part[b.pstride + ii[k]] = ay[k];
+        b.part[2 * b.pstride + ii[k]] = az[k];
+        b.part[3 * b.pstride + ii[k]] = ap[k];
     }
-    if (a.stats && lane == 0) {
-        atomicAdd(&a.stats[0], (unsigned long long)(t1 - t0));
-        atomicAdd(&a.stats[1], (unsigned long long)nearv);
+    if (stats && lane == 0) {
+        atomicAdd(&stats[0], (unsigned long long)(b.t1 - b.t0));
+        atomicAdd(&stats[1], (unsigned long long)nearv);
     }
 }
 
@@ -448,84 +452,36 @@ __device__ __forceinline__ void near_pair(int nk, double dx, double dy, double d
     }
 }
 
-// ---------------------------------------------------------- near kernels
-// One workgroup of NEAR_WAVES waves per (i-group, j-partition).  The waves
-// deal the group's flagged tiles round-robin among themselves, stage each
-// tile in their own LDS slice, and at the end combine their sums in a fixed
-// order (wave 0 + 1 + 2 + 3) before adding them to the far kernel's partials.
-constexpr int NEAR_WAVES = 4;
-
+// ------------------------------------------------------------ near group
+// One workgroup of NEAR_WAVES waves per i-group.  The waves deal the group's
+// flagged tiles round-robin among themselves, stage each tile in their own
+// LDS slice, and at the end combine their sums in a fixed order (wave
+// 0 + 1 + 2 + 3) before adding them to the far kernel's partials.
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
 
-// Fixed-order combination of the per-wave FP64 sums; adds them to the
-// partials of j-partition jp.
-template <int IPL>
-__device__ __forceinline__ void near_combine(const PairArgs &a, int jp, int w, int lane,
-                                             const int64_t (&ii)[IPL], const bool (&valid)[IPL],
-                                             double (&ax)[IPL], double (&ay)[IPL],
-                                             double (&az)[IPL], double (&ap)[IPL])
+__device__ __forceinline__ bool group_has_near(const Group &g)
 {
-    __shared__ double s_red[NEAR_WAVES - 1][4][IPL][64];
-    if (w > 0) {
-#pragma unroll
-        for (int k = 0; k < IPL; ++k) {
-            s_red[w - 1][0][k][lane] = ax[k];
-            s_red[w - 1][1][k][lane] = ay[k];
-            s_red[w - 1][2][k][lane] = az[k];
-            s_red[w - 1][3][k][lane] = ap[k];
-        }
-    }
-    __syncthreads();
-    if (w != 0) return;
-    double *p = partial_of(a, jp);
-#pragma unroll
-    for (int k = 0; k < IPL; ++k) {
-        for (int v = 0; v < NEAR_WAVES - 1; ++v) {
-            ax[k] += s_red[v][0][k][lane];
-            ay[k] += s_red[v][1][k][lane];
-            az[k] += s_red[v][2][k][lane];
-            ap[k] += s_red[v][3][k][lane];
-        }
-        if (!valid[k]) continue;
-        p[ii[k]] += ax[k];
-        p[a.ni + ii[k]] += ay[k];
-        p[2 * a.ni + ii[k]] += az[k];
-        p[3 * a.ni + ii[k]] += ap[k];
-    }
-}
-
-__device__ __forceinline__ bool group_has_near(const unsigned char *flags, int t0, int t1)
-{
-    for (int t = t0; t < t1; ++t)
-        if (flags[t]) return true;
+    for (int t = g.t0; t < g.t1; ++t)
+        if (g.flags[t - g.t0]) return true;
     return false;
 }
 
-// Near kernel: every pair of the flagged tiles in FP64 from FP64 positions
-// relative to the group's origin.  FAST (the FP32C product) seeds 1/sqrt(r^2)
-// with the FP32 hardware estimate plus one FP64 Newton step; the plain FP64
-// kernel uses the FP64 library rsqrt.
+struct NearLds {
+    double4 p[NEAR_WAVES][TJ];   // x, y, z relative to the origin; m
+    double2 h[NEAR_WAVES][TJ];   // signed H, original index (as double)
+};
+
 template <int IPL, bool FAST>
-__global__ __launch_bounds__(64 * NEAR_WAVES) void k_near(PairArgs a)
+__device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &J, int nk,
+                                           const Group &g, NearLds &lds, double *s_red)
 {
-    __shared__ double4 s_p[NEAR_WAVES][TJ];   // x, y, z relative to the origin; m
-    __shared__ double2 s_h[NEAR_WAVES][TJ];   // signed H, original index (as double)
-
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int64_t g = blockIdx.x % a.ngroups;
-    const int jp = (int)(blockIdx.x / a.ngroups);
-    int t0, t1;
-    tile_range(a, jp, t0, t1);
-    const unsigned char *flags = a.nearflag + g * a.ntiles;
-    if (!group_has_near(flags, t0, t1)) return;
-
-    const int64_t gb = g * (64 * IPL);
-    const int64_t io = min(gb + 32 * IPL, a.ni - 1);
-    const double Ox = a.i.x[io], Oy = a.i.y[io], Oz = a.i.z[io];
+    const int64_t io = min(g.gbase + 32 * IPL, g.iend - 1);
+    const double Ox = I.x[io], Oy = I.y[io], Oz = I.z[io];
 
     double xr[IPL], yr[IPL], zr[IPL], Hi[IPL], invHi[IPL];
     double ax[IPL], ay[IPL], az[IPL], ap[IPL];
@@ -534,42 +490,126 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near(PairArgs a)
     bool valid[IPL];
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
-        const int64_t i = gb + k * 64 + lane;
-        valid[k] = i < a.ni;
-        ii[k] = valid[k] ? i : a.ni - 1;
-        xr[k] = a.i.x[ii[k]] - Ox;
-        yr[k] = a.i.y[ii[k]] - Oy;
-        zr[k] = a.i.z[ii[k]] - Oz;
-        Hi[k] = fabs(a.i.Hs[ii[k]]);
+        const int64_t i = g.gbase + k * 64 + lane;
+        valid[k] = i < g.iend;
+        ii[k] = valid[k] ? i : g.iend - 1;
+        xr[k] = I.x[ii[k]] - Ox;
+        yr[k] = I.y[ii[k]] - Oy;
+        zr[k] = I.z[ii[k]] - Oz;
+        Hi[k] = fabs(I.Hs[ii[k]]);
         invHi[k] = 1.0 / Hi[k];
-        idi[k] = a.i.idx[ii[k]];
+        idi[k] = I.idx[ii[k]];
         ax[k] = ay[k] = az[k] = ap[k] = 0.0;
     }
 
     int cnt = 0;
-    for (int t = t0; t < t1; ++t) {
-        if (!flags[t]) continue;
+    for (int t = g.t0; t < g.t1; ++t) {
+        if (!g.flags[t - g.t0]) continue;
         if ((cnt++ % NEAR_WAVES) != w) continue;
         const int64_t j0 = (int64_t)t * TJ;
         wave_lds_sync();
         for (int q = 0; q < TJ / 64; ++q) {
             const int jj = q * 64 + lane;
             const int64_t j = j0 + jj;
-            s_p[w][jj] = make_double4(a.j.x[j] - Ox, a.j.y[j] - Oy, a.j.z[j] - Oz, a.j.m[j]);
-            s_h[w][jj] = make_double2(a.j.Hs[j], (double)a.j.idx[j]);
+            lds.p[w][jj] = make_double4(J.x[j] - Ox, J.y[j] - Oy, J.z[j] - Oz, J.m[j]);
+            lds.h[w][jj] = make_double2(J.Hs[j], (double)J.idx[j]);
         }
         wave_lds_sync();
         for (int jj = 0; jj < TJ; ++jj) {
-            const double4 p = s_p[w][jj];
-            const double2 h = s_h[w][jj];
+            const double4 p = lds.p[w][jj];
+            const double2 h = lds.h[w][jj];
 #pragma unroll
             for (int k = 0; k < IPL; ++k)
-                near_pair<FAST>(a.nkernel, p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k], invHi[k],
-                          fabs(h.x), p.w, h.x > 0.0, (int)h.y == idi[k], ax[k], ay[k], az[k],
-                          ap[k]);
+                near_pair<FAST>(nk, p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k], invHi[k],
+                                fabs(h.x), p.w, h.x > 0.0, (int)h.y == idi[k], ax[k], ay[k],
+                                az[k], ap[k]);
         }
     }
-    near_combine<IPL>(a, jp, w, lane, ii, valid, ax, ay, az, ap);
+
+    // fixed-order combination of the waves' sums, added to the partials
+    if (w > 0) {
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            double *r = s_red + (((w - 1) * 4) * IPL + k) * 64 + lane;
+            r[0] = ax[k];
+            r[IPL * 64] = ay[k];
+            r[2 * IPL * 64] = az[k];
+            r[3 * IPL * 64] = ap[k];
+        }
+    }
+    __syncthreads();
+    if (w != 0) return;
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        for (int v = 0; v < NEAR_WAVES - 1; ++v) {
+            const double *r = s_red + ((v * 4) * IPL + k) * 64 + lane;
+            ax[k] += r[0];
+            ay[k] += r[IPL * 64];
+            az[k] += r[2 * IPL * 64];
+            ap[k] += r[3 * IPL * 64];
+        }
+        if (!valid[k]) continue;
+        g.part[ii[k]] += ax[k];
+        g.part[g.pstride + ii[k]] += ay[k];
+        g.part[2 * g.pstride + ii[k]] += az[k];
+        g.part[3 * g.pstride + ii[k]] += ap[k];
+    }
+}
+
+// ======================================================= single-set kernels
+__device__ __forceinline__ void tile_range(const PairArgs &a, int jp, int &t0, int &t1)
+{
+    t0 = (int)((int64_t)a.ntiles * jp / a.jsplit);
+    t1 = (int)((int64_t)a.ntiles * (jp + 1) / a.jsplit);
+}
+
+template <int WG, int IPL>
+__device__ __forceinline__ Block single_block(const PairArgs &a)
+{
+    const int ib = blockIdx.x % a.niblocks;
+    const int jp = blockIdx.x / a.niblocks;
+    Block b;
+    b.ibase = (int64_t)ib * WG * IPL;
+    b.iend = a.ni;
+    tile_range(a, jp, b.t0, b.t1);
+    b.fstride = a.ntiles;
+    b.flags = a.nearflag + (int64_t)ib * (WG / 64) * a.ntiles + b.t0;
+    b.part = a.partial + (int64_t)jp * 4 * a.ni;
+    b.pstride = a.ni;
+    return b;
+}
+
+template <int WG, int IPL>
+__global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
+{
+    __shared__ float4 s_hi[TS];
+    __shared__ float4 s_lo[TS];
+    far_block_fp32c<WG, IPL>(a.i, a.j, a.tiles, single_block<WG, IPL>(a), a.stats, s_hi, s_lo);
+}
+
+template <int WG, int IPL>
+__global__ __launch_bounds__(WG) void k_far_fp64(PairArgs a)
+{
+    __shared__ double4 s_p[TS];
+    far_block_fp64<WG, IPL>(a.i, a.j, a.tiles, single_block<WG, IPL>(a), a.stats, s_p);
+}
+
+template <int IPL, bool FAST>
+__global__ __launch_bounds__(64 * NEAR_WAVES) void k_near(PairArgs a)
+{
+    __shared__ NearLds lds;
+    __shared__ double s_red[(NEAR_WAVES - 1) * 4 * IPL * 64];
+    const int64_t gi = blockIdx.x % a.ngroups;
+    const int jp = (int)(blockIdx.x / a.ngroups);
+    Group g;
+    g.gbase = gi * (64 * IPL);
+    g.iend = a.ni;
+    tile_range(a, jp, g.t0, g.t1);
+    g.flags = a.nearflag + gi * a.ntiles + g.t0;
+    g.part = a.partial + (int64_t)jp * 4 * a.ni;
+    g.pstride = a.ni;
+    if (g.gbase >= g.iend || !group_has_near(g)) return;
+    near_group<IPL, FAST>(a.i, a.j, a.nkernel, g, lds, s_red);
 }
 
 // Fixed-order FP64 sum of the j-partition partials, scattered to input order.
@@ -587,6 +627,89 @@ __global__ __launch_bounds__(256) void k_reduce_partials(PairArgs a)
     a.ay[o] = s[1];
     a.az[o] = s[2];
     a.pot[o] = s[3];
+}
+
+// ============================================================= batch kernels
+template <int WG, int IPL>
+__device__ __forceinline__ Block batch_block(const BatchArgs &a, int item)
+{
+    const int2 it = a.items[item];
+    const BatchSet st = a.sets[it.x];
+    Block b;
+    b.ibase = st.pofs + (int64_t)it.y * WG * IPL;
+    b.iend = st.pofs + st.n;
+    b.t0 = st.tile0;
+    b.t1 = st.tile0 + st.ntiles;
+    b.fstride = st.ntiles;
+    b.flags = a.nearflag + st.flagofs + (int64_t)it.y * (WG / 64) * st.ntiles;
+    b.part = a.partial;
+    b.pstride = a.padded;
+    return b;
+}
+
+// Persistent far kernel over a batch: every workgroup takes the next
+// (set, i-block) item from the queue until the queue is empty, so every
+// workgroup reaches the exit once all items are taken.
+template <int WG, int IPL, bool FP64>
+__global__ __launch_bounds__(WG) void k_far_batch(BatchArgs a)
+{
+    // one TS-entry FP64 tile, or the two TS-entry FP32 tiles (hi, lo)
+    __shared__ double4 s_buf[TS];
+    float4 *s_hi = reinterpret_cast<float4 *>(s_buf);
+    float4 *s_lo = s_hi + TS;
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= a.nitems) break;
+        const Block b = batch_block<WG, IPL>(a, item);
+        if (FP64)
+            far_block_fp64<WG, IPL>(a.j, a.j, a.tiles, b, a.stats, s_buf);
+        else
+            far_block_fp32c<WG, IPL>(a.j, a.j, a.tiles, b, a.stats, s_hi, s_lo);
+    }
+}
+
+template <int IPL, bool FAST>
+__global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_batch(BatchArgs a)
+{
+    __shared__ NearLds lds;
+    __shared__ double s_red[(NEAR_WAVES - 1) * 4 * IPL * 64];
+    const int64_t gg = blockIdx.x;
+    // the set this group belongs to (sets are ordered by group offset)
+    int lo = 0, hi = a.nsets - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (a.sets[mid].gofs <= gg) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchSet st = a.sets[lo];
+    const int64_t lg = gg - st.gofs;
+    Group g;
+    g.gbase = st.pofs + lg * (64 * IPL);
+    g.iend = st.pofs + st.n;
+    g.t0 = st.tile0;
+    g.t1 = st.tile0 + st.ntiles;
+    g.flags = a.nearflag + st.flagofs + lg * st.ntiles;
+    g.part = a.partial;
+    g.pstride = a.padded;
+    if (g.gbase >= g.iend || !group_has_near(g)) return;
+    near_group<IPL, FAST>(a.j, a.j, a.nkernel, g, lds, s_red);
+}
+
+// Scatter the per-slot sums of a batch back to input order.
+__global__ __launch_bounds__(256) void k_reduce_batch(BatchArgs a)
+{
+    const int64_t s = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (s >= a.padded) return;
+    const int o = a.j.idx[s];
+    if (o < 0) return;   // padding
+    a.ax[o] = a.partial[s];
+    a.ay[o] = a.partial[a.padded + s];
+    a.az[o] = a.partial[2 * a.padded + s];
+    a.pot[o] = a.partial[3 * a.padded + s];
 }
 
 inline dim3 grid_of(const PairArgs &a) { return dim3((unsigned)a.niblocks * (unsigned)a.jsplit); }
@@ -646,6 +769,76 @@ hipError_t launch_reduce_partials(const PairArgs &a, hipStream_t s)
 {
     if (a.ni <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((a.ni + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ batch launches
+template <int WG, int IPL, bool FP64>
+static hipError_t far_batch_launch(const BatchArgs &a, int nwg, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_far_batch<WG, IPL, FP64>), dim3(nwg), dim3(WG), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int WG, int IPL, bool FP64>
+static int far_batch_resident(int device)
+{
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_far_batch<WG, IPL, FP64>, WG, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    return per_cu * cus;
+}
+
+#define SG_FARB(WG, IPL)                                                                     \
+    if (wg == WG && ipl == IPL) {                                                            \
+        if (resident_only)                                                                   \
+            return precision == 0 ? far_batch_resident<WG, IPL, false>(device)               \
+                                  : far_batch_resident<WG, IPL, true>(device);               \
+        return (int)(precision == 0 ? far_batch_launch<WG, IPL, false>(a, nwg, s)            \
+                                    : far_batch_launch<WG, IPL, true>(a, nwg, s));           \
+    }
+
+static int far_batch_dispatch(int precision, const BatchArgs &a, int wg, int ipl, int nwg,
+                              hipStream_t s, bool resident_only, int device)
+{
+    SG_FARB(64, 1) SG_FARB(64, 2) SG_FARB(128, 1) SG_FARB(128, 2) SG_FARB(256, 1)
+    SG_FARB(256, 2)
+    return resident_only ? 0 : (int)hipErrorInvalidConfiguration;
+}
+
+int batch_resident_workgroups(int precision, int wg, int ipl, int device)
+{
+    BatchArgs dummy{};
+    return far_batch_dispatch(precision, dummy, wg, ipl, 0, nullptr, true, device);
+}
+
+bool batch_config_supported(int wg, int ipl)
+{
+    return (wg == 64 || wg == 128 || wg == 256) && (ipl == 1 || ipl == 2);
+}
+
+hipError_t launch_batch(int precision, const BatchArgs &a, int wg, int ipl, int nwg, hipStream_t s)
+{
+    if (a.padded <= 0) return hipSuccess;
+    hipError_t e = launch_fill_zero_int((int *)a.counter, 1, s);
+    if (e != hipSuccess) return e;
+    e = (hipError_t)far_batch_dispatch(precision, a, wg, ipl, nwg, s, false, 0);
+    if (e != hipSuccess) return e;
+    const dim3 block(64 * NEAR_WAVES), grid((unsigned)a.ngroups);
+    if (a.ngroups > 0) {
+        if (precision == 0) {
+            if (ipl == 1) hipLaunchKernelGGL((k_near_batch<1, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_near_batch<2, true>), grid, block, 0, s, a);
+        } else {
+            if (ipl == 1) hipLaunchKernelGGL((k_near_batch<1, false>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_near_batch<2, false>), grid, block, 0, s, a);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_reduce_batch, dim3((unsigned)((a.padded + 255) / 256)), dim3(256), 0, s,
+                       a);
     return hipGetLastError();
 }
 

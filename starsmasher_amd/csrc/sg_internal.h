@@ -52,12 +52,45 @@ struct PairArgs {
     unsigned long long *stats;    // [2]: tile visits, near visits (may be null)
 };
 
+// A batch of independent particle sets laid out back to back in one padded
+// sorted array; each set starts at a tile boundary.
+struct BatchSet {
+    int64_t pofs;      // first padded slot of the set (multiple of TJ)
+    int64_t n;         // real particles
+    int tile0;         // first tile
+    int ntiles;        // tiles of the set
+    int64_t gofs;      // first near group (groups of 64*ipl i)
+    int64_t flagofs;   // first near flag
+};
+
+struct BatchArgs {
+    SortedSet j;                 // padded sorted arrays of all sets (also the i-set)
+    const TileMeta *tiles;
+    const BatchSet *sets;
+    int nsets;
+    const int2 *items;           // (set, i-block), longest first
+    int nitems;
+    int64_t ngroups;             // near groups over all sets
+    int64_t padded;              // slots over all sets
+    int nkernel;
+    double *partial;             // [4][padded]
+    unsigned char *nearflag;
+    unsigned *counter;           // work-queue head
+    double *ax, *ay, *az, *pot;  // outputs, input order
+    unsigned long long *stats;
+};
+
 // Launchers (sg_direct.hip).  Return hipError_t of the launch.
 // An evaluation is far kernel -> near kernel -> reduce, in stream order.
 hipError_t launch_pair_far(int precision, const PairArgs &a, int wg, hipStream_t s);
 hipError_t launch_pair_near(int precision, const PairArgs &a, hipStream_t s);
 hipError_t launch_reduce_partials(const PairArgs &a, hipStream_t s);
 bool pair_config_supported(int precision, int wg, int ipl);
+// Batch: queue reset, persistent far kernel on nwg workgroups, near kernel,
+// scatter.  Contains no memset, so it can be captured in a graph.
+hipError_t launch_batch(int precision, const BatchArgs &a, int wg, int ipl, int nwg, hipStream_t s);
+int batch_resident_workgroups(int precision, int wg, int ipl, int device);
+bool batch_config_supported(int wg, int ipl);
 
 // Preparation (sg_prep.hip).
 struct PrepBuffers {
@@ -91,6 +124,16 @@ hipError_t launch_gather(const double *x, const double *y, const double *z,
 hipError_t launch_tile_meta(const double *sx, const double *sy, const double *sz,
                             const double *sH, int ntiles, TileMeta *tiles, hipStream_t s);
 hipError_t launch_fill_zero_int(int *p, int n, hipStream_t s);
+// Batch preparation: keys = (set << 42) | top 42 bits of the Morton code, and
+// the gather into the padded per-set layout.
+hipError_t launch_morton_batch(const double *x, const double *y, const double *z, int n,
+                               const int64_t *offsets, int nsets, const PrepBuffers &b,
+                               hipStream_t s);
+hipError_t launch_gather_batch(const double *x, const double *y, const double *z,
+                               const double *m, const double *r2, const int *perm,
+                               const int64_t *offsets, const BatchSet *sets, int nsets,
+                               int64_t padded, double *ox, double *oy, double *oz, double *om,
+                               double *oH, int *oidx, hipStream_t s);
 
 // Sorting (sg_sort.hip): radix sort of 63-bit keys with int payloads.
 hipError_t sort_pairs_temp_bytes(int n, size_t *bytes);

@@ -183,9 +183,96 @@ __global__ __launch_bounds__(TJ) void k_tile_meta(const double *__restrict__ sx,
     }
 }
 
+// Index of the set containing position p, for offsets[0..nsets] ascending.
+__device__ inline int set_of(const int64_t *__restrict__ offsets, int nsets, int64_t p)
+{
+    int lo = 0, hi = nsets - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (offsets[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Batch keys: the set index above the top 42 bits (14 per axis) of the
+// Morton code of the common bounding box, so a sort groups sets and orders
+// each along the curve.
+__global__ __launch_bounds__(RB) void k_morton_batch(const double *__restrict__ x,
+                                                     const double *__restrict__ y,
+                                                     const double *__restrict__ z, int n,
+                                                     const int64_t *__restrict__ offsets,
+                                                     int nsets, const double *__restrict__ bbox,
+                                                     uint64_t *__restrict__ keys,
+                                                     int *__restrict__ vals)
+{
+    const int k = blockIdx.x * RB + threadIdx.x;
+    if (k >= n) return;
+    const double ext = fmax(fmax(bbox[3] - bbox[0], bbox[4] - bbox[1]), bbox[5] - bbox[2]);
+    const double scale = ext > 0.0 ? 2097151.0 / ext : 0.0;
+    const uint64_t m = (spread3(quant21(x[k], bbox[0], scale)) << 2) |
+                       (spread3(quant21(y[k], bbox[1], scale)) << 1) |
+                       spread3(quant21(z[k], bbox[2], scale));
+    keys[k] = ((uint64_t)set_of(offsets, nsets, k) << 42) | (m >> 21);
+    vals[k] = k;
+}
+
+// Padded per-set layout: slot -> (set, rank within the set); ranks past the
+// set's count are massless copies of its last particle.
+__global__ __launch_bounds__(RB) void k_gather_batch(
+    const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+    const double *__restrict__ m, const double *__restrict__ r2, const int *__restrict__ perm,
+    const int64_t *__restrict__ offsets, const BatchSet *__restrict__ sets, int nsets,
+    int64_t padded, double *__restrict__ ox, double *__restrict__ oy, double *__restrict__ oz,
+    double *__restrict__ om, double *__restrict__ oH, int *__restrict__ oidx)
+{
+    const int64_t slot = blockIdx.x * (int64_t)RB + threadIdx.x;
+    if (slot >= padded) return;
+    int lo = 0, hi = nsets - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (sets[mid].pofs <= slot) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchSet st = sets[lo];
+    const int64_t rank = slot - st.pofs;
+    const bool real = rank < st.n;
+    const int src = perm[offsets[lo] + (real ? rank : st.n - 1)];
+    const double rr = r2[src];
+    const double H = sqrt(rr > 0.0 ? rr : -rr);
+    ox[slot] = x[src];
+    oy[slot] = y[src];
+    oz[slot] = z[src];
+    om[slot] = real ? m[src] : 0.0;
+    oH[slot] = (real && rr > 0.0) ? H : -H;
+    oidx[slot] = real ? src : -1;
+}
+
 inline int nblocks_for(int64_t n) { return (int)((n + RB - 1) / RB); }
 
 }  // namespace
+
+hipError_t launch_morton_batch(const double *x, const double *y, const double *z, int n,
+                               const int64_t *offsets, int nsets, const PrepBuffers &b,
+                               hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_morton_batch, dim3(nblocks_for(n)), dim3(RB), 0, s, x, y, z, n, offsets,
+                       nsets, b.bbox, b.keys_in, b.vals_in);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_batch(const double *x, const double *y, const double *z,
+                               const double *m, const double *r2, const int *perm,
+                               const int64_t *offsets, const BatchSet *sets, int nsets,
+                               int64_t padded, double *ox, double *oy, double *oz, double *om,
+                               double *oH, int *oidx, hipStream_t s)
+{
+    if (padded <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_batch, dim3(nblocks_for(padded)), dim3(RB), 0, s, x, y, z, m, r2,
+                       perm, offsets, sets, nsets, padded, ox, oy, oz, om, oH, oidx);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_zero_int(int *p, int n, hipStream_t s)
 {

@@ -201,6 +201,48 @@ class Gravity:
                                        *[C.c_void_p(p) for p in outs])
         check(st, "sg_compute_device")
 
+    # ------------------------------------------------------------ batch
+    def set_batch_config(self, precision, wg, ipl):
+        check(self._L.sg_set_batch_config(self._h, int(precision), int(wg), int(ipl)),
+              "sg_set_batch_config")
+
+    def get_batch_config(self, precision):
+        wg, ipl = C.c_int(), C.c_int()
+        check(self._L.sg_get_batch_config(self._h, int(precision), C.byref(wg), C.byref(ipl)),
+              "sg_get_batch_config")
+        return wg.value, ipl.value
+
+    def compute_batch_device(self, offsets, x, y, z, m, range2, ax, ay, az, pot, nkernel=0,
+                             precision=SG_PREC_FP32C, use_graph=True):
+        """Enqueue a batch of independent sets: set s is particles
+        offsets[s]..offsets[s+1]-1 of the device arrays."""
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        ntot = int(off[-1])
+        ins = [_dev_ptr(a, nm, ntot) for a, nm in ((x, "x"), (y, "y"), (z, "z"), (m, "m"),
+                                                    (range2, "range2"))]
+        outs = [_dev_ptr(a, nm, ntot) for a, nm in ((ax, "ax"), (ay, "ay"), (az, "az"),
+                                                     (pot, "pot"))]
+        st = self._L.sg_compute_batch_device(
+            self._h, off.size - 1, off.ctypes.data_as(C.POINTER(C.c_longlong)),
+            *[C.c_void_p(p) for p in ins], int(nkernel), int(precision), int(bool(use_graph)),
+            *[C.c_void_p(p) for p in outs])
+        check(st, "sg_compute_batch_device")
+
+    def compute_batch(self, sets, nkernel=0, precision=SG_PREC_FP32C, use_graph=False):
+        """Blocking host-array batch: `sets` is a list of dicts with x, y, z,
+        m, range2.  Returns a list of (ax, ay, az, pot) per set."""
+        from .hipmem import DeviceArray
+        sizes = [len(s["x"]) for s in sets]
+        off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        dev = [DeviceArray.from_host(np.concatenate([s[k] for s in sets]))
+               for k in ("x", "y", "z", "m", "range2")]
+        out = [DeviceArray(int(off[-1])) for _ in range(4)]
+        self.compute_batch_device(off, *dev, *out, nkernel=nkernel, precision=precision,
+                                  use_graph=use_graph)
+        self.synchronize()
+        res = [o.download() for o in out]
+        return [tuple(r[off[s]:off[s + 1]] for r in res) for s in range(len(sets))]
+
     def synchronize(self):
         check(self._L.sg_synchronize(self._h), "sg_synchronize")
 
