@@ -85,25 +85,146 @@ def rel_err(a, b):
     return ea, ep
 
 
+def profiled_traffic(wg, ipl):
+    """HBM bytes per launch of the far kernel for this tile shape, from the
+    committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "k_far_fp32c_giant1m.md")
+    try:
+        for ln in open(path):
+            if ln.startswith("traffic_bytes_per_launch"):
+                # traffic_bytes_per_launch = <bytes> (wg=<wg>, ipl=<ipl>)
+                val = float(ln.split("=")[1].split()[0])
+                if f"wg={wg}," in ln and f"ipl={ipl})" in ln:
+                    return val
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(d, seconds, nthreads):
     """The oracle's C restatement of the reference's symmetric CPU gravity
     loop (cpu_grav.f:1-381) on a bounded row sample of the same workload."""
     import oracle
     args = (d["x"], d["y"], d["z"], d["m"], d["hp"], d["u"])
     n = d["x"].size
-    # calibrate on 4 rows per thread
-    rows = 4 * nthreads
-    t = time.perf_counter()
-    ev, *_ = oracle.cpu_twin_rows_parallel(*args, 0, rows - 1, 0, nthreads)
-    dt = time.perf_counter() - t
-    rows = int(max(rows, min(n - 1, rows * seconds / max(dt, 1e-3))))
-    t = time.perf_counter()
-    ev, *_ = oracle.cpu_twin_rows_parallel(*args, 0, rows - 1, 0, nthreads)
-    dt = time.perf_counter() - t
+
+    def run(rows):
+        t = time.perf_counter()
+        if nthreads == 1:
+            ev, *_ = oracle.cpu_twin_rows(*args, 0, rows - 1, 0)
+        else:
+            ev, *_ = oracle.cpu_twin_rows_parallel(*args, 0, rows - 1, 0, nthreads)
+        return ev, time.perf_counter() - t
+
+    # two-point calibration: the parallel form has an O(threads N) set-up
+    r1, r2 = 2 * nthreads, 8 * nthreads
+    _, t1 = run(r1)
+    _, t2 = run(r2)
+    per_row = max((t2 - t1) / (r2 - r1), 1e-6)
+    rows = int(max(r2, min(n - 1, r1 + (seconds - t1) / per_row)))
+    ev, dt = run(rows)
     # each symmetric evaluation covers two ordered pair-interactions (i<-j, j<-i)
     return {"value": 2.0 * ev / dt / 1e9, "unit": "Gpairs/s", "cores": nthreads, "kind": "port",
             "sample": f"rows 1..{rows} of the symmetric j>=i loop over all {n} particles "
                       f"({ev} evaluations, {dt:.1f} s)"}
+
+
+def bench_batch(args):
+    """64 independent stars of log-uniform N in [2048, 131072] (even entries
+    Plummer spheres, odd entries n = 1.5 polytropes) in one persistent,
+    graph-captured launch, against 64 separate single-set launches."""
+    from starsmasher_amd import SG_PREC_FP32C, SG_PREC_FP64, Gravity, ics
+    from starsmasher_amd.autotune import tune_batch
+    from starsmasher_amd.hipmem import DeviceArray, runtime
+
+    log("bench: generating 64-star batch")
+    stars = ics.batch_set(64)
+    sizes = np.array([s["x"].size for s in stars], dtype=np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    ntot = int(off[-1])
+    pairs = float((sizes.astype(np.float64) ** 2).sum())
+    g = Gravity(0)
+    dev = [DeviceArray.from_host(np.concatenate([s[k] for s in stars]))
+           for k in ("x", "y", "z", "m", "range2")]
+    out = [DeviceArray(ntot) for _ in range(4)]
+    tuning = {}
+    if args.tune:
+        for prec in (SG_PREC_FP32C, SG_PREC_FP64):
+            tr = tune_batch(g, off, dev, out, prec, reps=1, log=log)
+            tuning["fp32c" if prec == SG_PREC_FP32C else "fp64"] = {
+                "best": list(tr.best), "table_ms": [[r[0], r[1], round(r[3], 2)] for r in tr.table]}
+
+    def run_batch(prec, steps, graph=True):
+        g.compute_batch_device(off, *dev, *out, precision=prec, use_graph=graph)
+        g.synchronize()
+        runtime().hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.compute_batch_device(off, *dev, *out, precision=prec, use_graph=graph)
+            g.synchronize()
+        runtime().hipDeviceSynchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, [o.download() for o in out]
+
+    def run_separate(prec, steps):
+        for _ in range(steps + 1):
+            if _ == 1:
+                runtime().hipDeviceSynchronize()
+                t0 = time.perf_counter()
+            for s in range(64):
+                a, b = int(off[s]), int(off[s + 1])
+                g.compute_device(b - a, *[x.ptr + 8 * a for x in dev], *[o.ptr + 8 * a for o in out],
+                                 precision=prec)
+            g.synchronize()
+        runtime().hipDeviceSynchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, [o.download() for o in out]
+
+    log("bench: batch FP32C")
+    ms32, r32 = run_batch(SG_PREC_FP32C, args.steps)
+    log(f"bench: batch FP32C {ms32:.2f} ms/step")
+    ms32_eager, _ = run_batch(SG_PREC_FP32C, args.steps, graph=False)
+    ms32_sep, s32 = run_separate(SG_PREC_FP32C, args.steps)
+    log(f"bench: 64 separate launches FP32C {ms32_sep:.2f} ms/step")
+    ms64, r64 = run_batch(SG_PREC_FP64, max(1, args.steps // 2))
+    ms64_sep, _ = run_separate(SG_PREC_FP64, max(1, args.steps // 2))
+
+    per_star = []
+    import oracle
+    rng = np.random.default_rng(0)
+    for s in range(64):
+        a, b = int(off[s]), int(off[s + 1])
+        ea, ep = rel_err([x[a:b] for x in r32], [x[a:b] for x in r64])
+        rows = rng.choice(b - a, min(16, b - a), replace=False)
+        st = stars[s]
+        ref = oracle.direct(st["x"], st["y"], st["z"], st["m"], st["range2"], nkernel=0, rows=rows,
+                            nthreads=min(16, os.cpu_count() or 1))
+        eo, po = rel_err([x[a:b][rows] for x in r32], ref)
+        es, _ = rel_err([x[a:b] for x in s32], [x[a:b] for x in r64])
+        per_star.append({"n": b - a, "acc_max_vs_fp64": float(ea.max()),
+                         "pot_max_vs_fp64": float(ep.max()),
+                         "acc_max_vs_oracle_rows": float(eo.max()),
+                         "pot_max_vs_oracle_rows": float(po.max()),
+                         "separate_acc_max_vs_fp64": float(es.max())})
+    line = {
+        "metric": "G pair-interactions/s, 64 independent stars in one graph-captured persistent "
+                  "launch (FP32+comp)",
+        "value": round(pairs / (ms32 * 1e-3) / 1e9, 3), "unit": "Gpairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": round(ms32, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "64 stars, N log-uniform in [2048, 131072] (seed 4), Plummer / "
+                               "n=1.5 polytrope alternating, nkernel=0",
+                   "n_total": ntot, "sum_n2": pairs,
+                   "tile": dict(zip(("wg", "ipl"), g.get_batch_config(SG_PREC_FP32C)))},
+        "batch_eager_ms": round(ms32_eager, 3),
+        "separate_launches_ms": round(ms32_sep, 3),
+        "batch_speedup_vs_separate": round(ms32_sep / ms32, 3),
+        "fp64_batch_ms": round(ms64, 3), "fp64_separate_ms": round(ms64_sep, 3),
+        "fp32c_over_fp64": round(ms64 / ms32, 3),
+        "per_star_worst": {k: max(p[k] for p in per_star) for k in per_star[0] if k != "n"},
+        "per_star": per_star,
+        "autotune": tuning or None,
+    }
+    g.close()
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -131,6 +252,8 @@ def main():
     from starsmasher_amd import SG_PREC_FP32C, SG_PREC_FP64, Gravity
     from starsmasher_amd.hipmem import DeviceArray
 
+    if args.workload == "batch64":
+        return bench_batch(args)
     log(f"bench: generating workload {args.workload}")
     d, desc = make_workload(args.workload)
     n = d["x"].size
@@ -195,11 +318,18 @@ def main():
                 "acc_median": float(np.median(ea_o)), "pot_max": float(ep_o.max()),
                 "pot_median": float(np.median(ep_o))}
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu:
         nth = min(16, os.cpu_count() or 1)
         log(f"bench: CPU baseline on {nth} threads")
         cpu = cpu_baseline(d, args.cpu_seconds, nth)
+        log("bench: CPU baseline on 1 thread")
+        cpu1 = cpu_baseline(d, args.cpu_seconds / 3, 1)
+        try:
+            cpu["cpu_model"] = next(ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo")
+                                    if ln.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gpairs/s", "n_gpus": 1,
@@ -210,7 +340,7 @@ def main():
                    "jsplit": js}},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None, "kernel": "k_far_fp32c",
+                     "traffic": profiled_traffic(wg, ipl), "kernel": "k_far_fp32c",
                      "flops_per_pair": FLOPS_PER_PAIR,
                      "far_pairs_per_launch": far_pairs, "far_kernel_ms": round(r32["far_ms"], 3)},
         "phases_ms": {"prep": round(r32["prep_ms"], 3), "far": round(r32["far_ms"], 3),
@@ -218,6 +348,7 @@ def main():
         "near_tile_fraction": round(near_frac, 5),
         "accuracy": acc,
         "cpu_baseline": cpu,
+        "cpu_baseline_1core": cpu1,
         "autotune": tuning or None,
     }
     if r64 is not None:

@@ -58,3 +58,36 @@ def tune(g, n, inputs, outputs, precision=SG_PREC_FP32C, nkernel=0, candidates=N
                 best, best_ms = (wg, ipl, js), row[3]
     g.set_config(precision, *best)
     return TuneResult(precision, best, best_ms, table)
+
+
+BATCH_CANDIDATES = [(64, 1), (64, 2), (128, 1), (128, 2), (256, 1), (256, 2)]
+
+
+def tune_batch(g, offsets, inputs, outputs, precision=SG_PREC_FP32C, nkernel=0,
+               candidates=None, reps=2, log=None) -> TuneResult:
+    """Time each (workgroup size, i per lane) shape of the persistent batch
+    kernel on the given device-resident batch (graph replays) and install the
+    fastest with ``sg_set_batch_config``."""
+    cands = candidates or BATCH_CANDIDATES
+    table = []
+    best, best_ms = None, float("inf")
+    for wg, ipl in cands:
+        g.set_batch_config(precision, wg, ipl)
+        g.compute_batch_device(offsets, *inputs, *outputs, nkernel=nkernel, precision=precision,
+                               use_graph=True)
+        g.synchronize()
+        tot = 0.0
+        for _ in range(reps):
+            g.compute_batch_device(offsets, *inputs, *outputs, nkernel=nkernel,
+                                   precision=precision, use_graph=True)
+            g.synchronize()
+            p, f, nr = g.last_timings()
+            tot += p + f + nr
+        row = (wg, ipl, 0, tot / reps, 0.0, 0.0)
+        table.append(row)
+        if log:
+            log(f"autotune batch prec={precision} wg={wg} ipl={ipl}: {row[3]:.2f} ms")
+        if row[3] < best_ms:
+            best, best_ms = (wg, ipl), row[3]
+    g.set_batch_config(precision, *best)
+    return TuneResult(precision, best, best_ms, table)
