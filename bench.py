@@ -245,7 +245,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the gravity step shards by i-range with replicated j; see DESIGN.md)",
+              "(several GPUs would split the i-range with the j-set replicated on each; "
+              "only the one-GPU share is built and measured)",
               file=sys.stderr)
         sys.exit(2)
 

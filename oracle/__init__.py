@@ -238,8 +238,8 @@ def ref_cpu_gravity(x, y, z, am, hp, u, nkernel=0, ngrav_lower=1, ngrav_upper=No
 
 
 # --------------------------------------------------------------------------
-# Independent pure-numpy restatement for tiny inputs (closed forms of
-# SURVEY appendix A restated from grav_force_direct.cu:55-182; cubic only).
+# Independent pure-numpy restatement for tiny inputs (the cubic-spline
+# closed forms of grav_force_direct.cu:55-182; cubic only).
 # --------------------------------------------------------------------------
 def _cubic_A(q):
     q = np.asarray(q, dtype=np.float64)

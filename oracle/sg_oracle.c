@@ -17,9 +17,11 @@
  *
  * Pinning (what checks each function; see tests/test_oracle.py):
  *   sgo_direct        golden grpot of the reference's own start dumps (cubic),
- *                     brute force of 2-3 particle cases against scipy quad-free
- *                     closed forms, -grad(phi) == a by central differences,
- *                     momentum conservation, Newtonian limit, q=1 continuity.
+ *                     an independent numpy restatement on small sets,
+ *                     -grad(phi) == a by central differences, momentum
+ *                     conservation, Newtonian limit, q=1 continuity.
+ *   sgo_kernel_AP     the potential of each smoothing density integrated by
+ *                     scipy quadrature; A and P continuous at q = 1.
  *   sgo_cpu_twin      the reference's own cpu_grav.f compiled from its source
  *                     (oracle/_ref), bit-for-bit modulo summation order.
  *   sgo_vjp           central finite differences of sgo_loss_particle and

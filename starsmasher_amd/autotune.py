@@ -3,8 +3,11 @@
 For a precision and a particle set already resident on the device, times
 every supported (workgroup size, i-particles per lane, j-partitions) shape of
 the far kernel and keeps the fastest whole evaluation (prep + far + near +
-reduce, HIP events).  The chosen shape is installed on the context with
-``sg_set_config`` and returned with the full timing table.
+reduce, HIP events).  The shape installed when tuning starts stays unless
+another is faster by more than ``margin`` (default 2%), so that run-to-run
+noise does not flip between shapes of equal speed.  The chosen shape is
+installed on the context with ``sg_set_config`` and returned with the full
+timing table.
 """
 from __future__ import annotations
 
@@ -26,16 +29,25 @@ class TuneResult:
     table: list = field(default_factory=list)   # (wg, ipl, jsplit, total_ms, far_ms, near_ms)
 
 
+def _pick(table, incumbent, margin):
+    """Fastest row, unless the incumbent shape is within ``margin`` of it."""
+    best = min(table, key=lambda r: r[3])
+    for r in table:
+        if tuple(r[:len(incumbent)]) == tuple(incumbent) and r[3] <= best[3] * (1.0 + margin):
+            return r
+    return best
+
+
 def tune(g, n, inputs, outputs, precision=SG_PREC_FP32C, nkernel=0, candidates=None,
-         jsplits=(0,), reps=2, log=None) -> TuneResult:
+         jsplits=(0,), reps=2, margin=0.02, log=None) -> TuneResult:
     """Time each candidate shape on the given device-resident set.
 
     g: a :class:`starsmasher_amd.Gravity`; inputs: 5 device arrays
     (x, y, z, m, range2); outputs: 4 device arrays of length n.
     """
     cands = candidates or CANDIDATES[precision]
+    incumbent = tuple(g.get_config(precision))
     table = []
-    best, best_ms = None, float("inf")
     for wg, ipl in cands:
         for js in jsplits:
             g.set_config(precision, wg, ipl, js)
@@ -54,23 +66,23 @@ def tune(g, n, inputs, outputs, precision=SG_PREC_FP32C, nkernel=0, candidates=N
             if log:
                 log(f"autotune prec={precision} wg={wg} ipl={ipl} jsplit={js}: "
                     f"{row[3]:.2f} ms (far {row[4]:.2f}, near {row[5]:.2f})")
-            if row[3] < best_ms:
-                best, best_ms = (wg, ipl, js), row[3]
+    pick = _pick(table, incumbent, margin)
+    best = tuple(pick[:3])
     g.set_config(precision, *best)
-    return TuneResult(precision, best, best_ms, table)
+    return TuneResult(precision, best, pick[3], table)
 
 
 BATCH_CANDIDATES = [(64, 1), (64, 2), (128, 1), (128, 2), (256, 1), (256, 2)]
 
 
 def tune_batch(g, offsets, inputs, outputs, precision=SG_PREC_FP32C, nkernel=0,
-               candidates=None, reps=2, log=None) -> TuneResult:
+               candidates=None, reps=2, margin=0.02, log=None) -> TuneResult:
     """Time each (workgroup size, i per lane) shape of the persistent batch
     kernel on the given device-resident batch (graph replays) and install the
     fastest with ``sg_set_batch_config``."""
     cands = candidates or BATCH_CANDIDATES
+    incumbent = tuple(g.get_batch_config(precision))
     table = []
-    best, best_ms = None, float("inf")
     for wg, ipl in cands:
         g.set_batch_config(precision, wg, ipl)
         g.compute_batch_device(offsets, *inputs, *outputs, nkernel=nkernel, precision=precision,
@@ -87,7 +99,7 @@ def tune_batch(g, offsets, inputs, outputs, precision=SG_PREC_FP32C, nkernel=0,
         table.append(row)
         if log:
             log(f"autotune batch prec={precision} wg={wg} ipl={ipl}: {row[3]:.2f} ms")
-        if row[3] < best_ms:
-            best, best_ms = (wg, ipl), row[3]
+    pick = _pick(table, incumbent, margin)
+    best = tuple(pick[:2])
     g.set_batch_config(precision, *best)
-    return TuneResult(precision, best, best_ms, table)
+    return TuneResult(precision, best, pick[3], table)

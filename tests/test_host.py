@@ -1,0 +1,30 @@
+"""Host-side logic that needs no GPU: the autotuner's choice rule and the
+bench's reading of the committed traffic profile."""
+import importlib.util
+import os
+
+from starsmasher_amd.autotune import _pick
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_pick_keeps_incumbent_within_margin():
+    table = [(64, 1, 0, 100.0), (256, 1, 0, 100.5), (128, 1, 0, 99.0)]
+    assert _pick(table, (256, 1, 0), 0.02)[:3] == (256, 1, 0)     # 100.5 <= 99 * 1.02
+    assert _pick(table, (256, 1, 0), 0.01)[:3] == (128, 1, 0)     # 100.5 >  99 * 1.01
+    assert _pick(table, (64, 2, 0), 0.02)[:3] == (128, 1, 0)      # incumbent not in table
+    assert _pick(table, (256, 1), 0.02)[:2] == (256, 1)           # batch rows: (wg, ipl)
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_profiled_traffic_reads_committed_profile():
+    b = _bench()
+    t = b.profiled_traffic(256, 1)
+    assert t is not None and t > 1e9
+    assert b.profiled_traffic(64, 4) is None
