@@ -168,6 +168,37 @@ int sg_last_timings(sg_context *ctx, float *prep_ms, float *far_ms, float *near_
  * how many took the softened (near-field) path. */
 int sg_last_tile_stats(sg_context *ctx, long long *visits, long long *near_visits);
 
+/* Adjoint (no reference counterpart; the reference has no derivative of the
+ * gravity step beyond kernels.f:215-279's dphi/dh, which the H-gradient here
+ * generalises).  For the forward outputs over all n particles (ibeg = 0,
+ * ni = n) and cotangents w = (wx, wy, wz), u of length n, computes the
+ * gradient of
+ *     L = sum_i wx_i ax_i + wy_i ay_i + wz_i az_i + u_i pot_i
+ * with respect to x_i, y_i, z_i, m_i and H_i = sqrt|range2_i| (the support
+ * radius; dL/dhp = 2 dL/dH for range2 = +-4 hp^2).  The r = 0 rule is the
+ * forward one: the self pair contributes through its potential term, a
+ * distinct coincident pair through the r -> 0 limit (the force factor F(0)
+ * is finite, so its x-gradient is F(0) (m_k w_l - m_l w_k)).
+ * precision SG_PREC_FP32C: Newtonian far pairs in FP32 with hi/lo positions
+ * and FP64 folding, softened pairs in FP64; SG_PREC_FP64: all FP64.
+ * Device pointers, asynchronous on the context stream like
+ * sg_compute_device (sg_synchronize reports validation, including
+ * non-finite cotangents as SG_ERR_NONFINITE); sg_last_timings covers it.
+ * The tile shape is the forward far kernel's workgroup size for the
+ * precision (one i per lane). */
+int sg_vjp_device(sg_context *ctx, int n,
+                  const double *x, const double *y, const double *z,
+                  const double *m, const double *range2, int nkernel, int precision,
+                  const double *wx, const double *wy, const double *wz, const double *u,
+                  double *gx, double *gy, double *gz, double *gm, double *gH);
+
+/* Host-pointer adjoint (blocking), same semantics as sg_vjp_device. */
+int sg_vjp_host(sg_context *ctx, int n,
+                const double *x, const double *y, const double *z,
+                const double *m, const double *range2, int nkernel, int precision,
+                const double *wx, const double *wy, const double *wz, const double *u,
+                double *gx, double *gy, double *gz, double *gm, double *gH);
+
 /* Legacy-layer equivalents with status returns (used by layer 1). */
 int sg_init_device(int device);
 int sg_firsthalf(int n, int n_lower, int my_length,

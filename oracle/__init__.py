@@ -82,9 +82,15 @@ def lib():
         L.sgo_vjp.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp,
                               _dp, _dp, _dp, _dp, _dp, C.c_int]
         L.sgo_vjp.restype = C.c_int
+        L.sgo_vjp_rows.argtypes = [C.c_int, _i64p, C.c_int64, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]
+        L.sgo_vjp_rows.restype = C.c_int
         L.sgo_loss_particle.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int,
                                         _dp, _dp, _dp, _dp]
         L.sgo_loss_particle.restype = C.c_double
+        L.sgo_fd_grad.argtypes = [C.c_int, _i64p, C.c_int64, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                                  _dp, _dp, _dp, _dp, C.c_double, _dp, C.c_int]
+        L.sgo_fd_grad.restype = C.c_int
         L.sgo_kernel_AP.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.sgo_kernel_dAP.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.sgo_epot.argtypes = [C.c_int64, _dp, _dp]
@@ -180,16 +186,25 @@ def partition(n: int, nranks: int):
     return lo, hi
 
 
-def vjp(x, y, z, m, range2, w, u, nkernel=0, nthreads=0):
-    """FP64 gradient of L = sum w_i.a_i + u_i phi_i w.r.t. (x, y, z, m, H)."""
+def vjp(x, y, z, m, range2, w, u, nkernel=0, nthreads=0, rows=None):
+    """FP64 gradient of L = sum w_i.a_i + u_i phi_i w.r.t. (x, y, z, m, H),
+    for all particles or the 0-based ``rows`` (outputs indexed like rows)."""
     x, y, z, m, range2 = map(_f64, (x, y, z, m, range2))
     w = np.asarray(w, dtype=np.float64)
     wx, wy, wz = (np.ascontiguousarray(w[:, k]) for k in range(3))
     u = _f64(u)
     n = x.size
+    nt = int(nthreads or os.cpu_count() or 1)
+    if rows is not None:
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        out = [np.empty(rows.size) for _ in range(5)]
+        rc = lib().sgo_vjp_rows(n, rows, rows.size, x, y, z, m, range2, nkernel, wx, wy, wz, u,
+                                *out, nt)
+        if rc != 0:
+            raise ValueError("sgo_vjp_rows rejected its arguments")
+        return tuple(out)
     out = [np.empty(n) for _ in range(5)]
-    rc = lib().sgo_vjp(n, x, y, z, m, range2, nkernel, wx, wy, wz, u, *out,
-                       int(nthreads or os.cpu_count() or 1))
+    rc = lib().sgo_vjp(n, x, y, z, m, range2, nkernel, wx, wy, wz, u, *out, nt)
     if rc != 0:
         raise ValueError("sgo_vjp rejected its arguments")
     return tuple(out)
@@ -200,6 +215,21 @@ def loss_particle(k, x, y, z, m, range2, w, u, nkernel=0):
     w = np.asarray(w, dtype=np.float64)
     wx, wy, wz = (np.ascontiguousarray(w[:, c]) for c in range(3))
     return lib().sgo_loss_particle(x.size, int(k), x, y, z, m, range2, nkernel, wx, wy, wz, _f64(u))
+
+
+def fd_grad(ks, x, y, z, m, range2, w, u, nkernel=0, step=1e-6, nthreads=0):
+    """Central finite differences of L w.r.t. (x, y, z, m, H) of the
+    particles ks; returns an array of shape (len(ks), 5)."""
+    x, y, z, m, range2 = map(_f64, (x, y, z, m, range2))
+    w = np.asarray(w, dtype=np.float64)
+    wx, wy, wz = (np.ascontiguousarray(w[:, c]) for c in range(3))
+    ks = np.ascontiguousarray(ks, dtype=np.int64)
+    out = np.empty((ks.size, 5))
+    rc = lib().sgo_fd_grad(x.size, ks, ks.size, x, y, z, m, range2, nkernel, wx, wy, wz, _f64(u),
+                           float(step), out.reshape(-1), int(nthreads or os.cpu_count() or 1))
+    if rc != 0:
+        raise ValueError("sgo_fd_grad rejected its arguments")
+    return out
 
 
 def epot(m, pot):

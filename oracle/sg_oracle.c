@@ -26,6 +26,8 @@
  *                     (oracle/_ref), bit-for-bit modulo summation order.
  *   sgo_vjp           central finite differences of sgo_loss_particle and
  *                     kernels.f dphidh (cubic) / dphi332hdh / dphic42hdh.
+ *   sgo_fd_grad       the same central differences; its loss is pinned
+ *                     through sgo_direct (the row/column sums it adds).
  *   sgo_partition     hand-worked splits and the invariants of main.f:297-349.
  */
 #include <math.h>
@@ -558,23 +560,28 @@ static void pair_loss_grad(int nkernel, double dx, double dy, double dz,
 }
 
 /*
- * Full gradient of L.  O(N^2) gather form: for every particle k, sum over the
- * pairs where k is the i-side and where k is the j-side.  gx..gH have length n.
+ * Gradient of L for the particles rows[0..nrows) (outputs indexed by row
+ * position).  O(nrows N) gather form: for every particle k, sum over the
+ * pairs where k is the i-side and where k is the j-side.
  * Self pairs (i == j) contribute phi_self = -c flag (2 m / H): dL/dm and dL/dH.
  * Distinct coincident pairs (r == 0) contribute their r->0 limit.
  */
-int sgo_vjp(int n, const double *x, const double *y, const double *z,
-            const double *m, const double *range2, int nkernel,
-            const double *wx, const double *wy, const double *wz, const double *u,
-            double *gxo, double *gyo, double *gzo, double *gmo, double *gHo, int nthreads)
+int sgo_vjp_rows(int n, const int64_t *rows, int64_t nrows,
+                 const double *x, const double *y, const double *z,
+                 const double *m, const double *range2, int nkernel,
+                 const double *wx, const double *wy, const double *wz, const double *u,
+                 double *gxo, double *gyo, double *gzo, double *gmo, double *gHo, int nthreads)
 {
-    if (n < 0 || !check_kernel(nkernel)) return -1;
+    if (n < 0 || nrows < 0 || !check_kernel(nkernel)) return -1;
+    for (int64_t r = 0; r < nrows; ++r)
+        if (rows[r] < 0 || rows[r] >= n) return -1;
     const double c = self_coeff(nkernel);
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 16)
 #endif
-    for (int64_t k = 0; k < n; ++k) {
+    for (int64_t r = 0; r < nrows; ++r) {
+        const int64_t k = rows[r];
         const double Hk = decode_H(range2[k]);
         double gx = 0, gy = 0, gz = 0, gm = 0, gH = 0;
         for (int64_t l = 0; l < n; ++l) {
@@ -592,6 +599,14 @@ int sgo_vjp(int n, const double *x, const double *y, const double *z,
                     /* k is j, l is i: pot_l += -c*(m_k/H_l + m_k/H_k) */
                     gm += u[l] * (-c * sk) * (1.0 / Hl + 1.0 / Hk);
                     gH += u[l] * (-c * sk) * (-m[k] / (Hk * Hk));
+                    /* a_k = m_l F dr and a_l = m_k F (-dr) with F finite at r = 0:
+                     * d(w_k.a_k + w_l.a_l)/dx_k = F(0) (m_k w_l - m_l w_k) */
+                    double A0, P0;
+                    kernel_AP(nkernel, 0.0, 0.0, &A0, &P0);
+                    const double F0 = 0.5 * A0 * (1.0 / (Hk * Hk * Hk) + 1.0 / (Hl * Hl * Hl));
+                    gx += F0 * (m[k] * wx[l] - m[l] * wx[k]);
+                    gy += F0 * (m[k] * wy[l] - m[l] * wy[k]);
+                    gz += F0 * (m[k] * wz[l] - m[l] * wz[k]);
                 } else {
                     gm += u[k] * (-c * sk) * (2.0 / Hk);
                 }
@@ -610,34 +625,107 @@ int sgo_vjp(int n, const double *x, const double *y, const double *z,
             gm += gmj;
             gH += gHj;
         }
-        gxo[k] = gx; gyo[k] = gy; gzo[k] = gz; gmo[k] = gm; gHo[k] = gH;
+        gxo[r] = gx; gyo[r] = gy; gzo[r] = gz; gmo[r] = gm; gHo[r] = gH;
     }
     return 0;
 }
 
+int sgo_vjp(int n, const double *x, const double *y, const double *z,
+            const double *m, const double *range2, int nkernel,
+            const double *wx, const double *wy, const double *wz, const double *u,
+            double *gxo, double *gyo, double *gzo, double *gmo, double *gHo, int nthreads)
+{
+    if (n < 0) return -1;
+    int64_t *rows = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    if (!rows) return -1;
+    for (int64_t k = 0; k < n; ++k) rows[k] = k;
+    const int rc = sgo_vjp_rows(n, rows, n, x, y, z, m, range2, nkernel, wx, wy, wz, u,
+                                gxo, gyo, gzo, gmo, gHo, nthreads);
+    free(rows);
+    return rc;
+}
+
 /* The part of L that depends on particle k's (x, m, H): its own row plus its
- * column.  Finite differences of this against sgo_vjp pin the adjoint. */
+ * column, with particle k's position, mass and range2 replaced by the given
+ * values (the arrays are not modified). */
+static double loss_particle_at(int n, int k, double xk, double yk, double zk, double mk,
+                               double r2k, const double *x, const double *y, const double *z,
+                               const double *m, const double *range2, int nkernel,
+                               const double *wx, const double *wy, const double *wz,
+                               const double *u)
+{
+    const double Hk = decode_H(r2k), fk = decode_flag(r2k);
+    double L = 0.0;
+    for (int64_t l = 0; l < n; ++l) {
+        pairsum row = {0, 0, 0, 0}, col = {0, 0, 0, 0};
+        if (l == k) {
+            pair_direct(nkernel, 0, 1, 0.0, 0.0, 0.0, Hk, Hk, mk, mk, fk, &row);
+            L += wx[k] * row.ax + wy[k] * row.ay + wz[k] * row.az + u[k] * row.pot;
+            continue;
+        }
+        const double Hl = decode_H(range2[l]);
+        /* row: i = k, j = l */
+        pair_direct(nkernel, 0, 0, x[l] - xk, y[l] - yk, z[l] - zk, Hk, Hl, mk, m[l],
+                    decode_flag(range2[l]), &row);
+        L += wx[k] * row.ax + wy[k] * row.ay + wz[k] * row.az + u[k] * row.pot;
+        /* column: i = l, j = k */
+        pair_direct(nkernel, 0, 0, xk - x[l], yk - y[l], zk - z[l], Hl, Hk, m[l], mk, fk, &col);
+        L += wx[l] * col.ax + wy[l] * col.ay + wz[l] * col.az + u[l] * col.pot;
+    }
+    return L;
+}
+
+/* Finite differences of this against sgo_vjp pin the adjoint. */
 double sgo_loss_particle(int n, int k, const double *x, const double *y, const double *z,
                          const double *m, const double *range2, int nkernel,
                          const double *wx, const double *wy, const double *wz, const double *u)
 {
     if (k < 0 || k >= n || !check_kernel(nkernel)) return NAN;
-    double L = 0.0;
-    for (int64_t l = 0; l < n; ++l) {
-        pairsum row = {0, 0, 0, 0}, col = {0, 0, 0, 0};
-        /* row: i = k, j = l */
-        pair_direct(nkernel, 0, l == k, x[l] - x[k], y[l] - y[k], z[l] - z[k],
-                    decode_H(range2[k]), decode_H(range2[l]), m[k], m[l],
-                    decode_flag(range2[l]), &row);
-        L += wx[k] * row.ax + wy[k] * row.ay + wz[k] * row.az + u[k] * row.pot;
-        if (l == k) continue;
-        /* column: i = l, j = k */
-        pair_direct(nkernel, 0, 0, x[k] - x[l], y[k] - y[l], z[k] - z[l],
-                    decode_H(range2[l]), decode_H(range2[k]), m[l], m[k],
-                    decode_flag(range2[k]), &col);
-        L += wx[l] * col.ax + wy[l] * col.ay + wz[l] * col.az + u[l] * col.pot;
+    return loss_particle_at(n, k, x[k], y[k], z[k], m[k], range2[k], x, y, z, m, range2,
+                            nkernel, wx, wy, wz, u);
+}
+
+/* Central differences of L for particles ks[0..nks): out[5 r + c] for
+ * c = x, y, z, m, H.  Steps: step * max(1, |v|) for x, y, z, m; step on H
+ * (range2 rescaled keeping its sign).  OpenMP over the sampled particles. */
+int sgo_fd_grad(int n, const int64_t *ks, int64_t nks, const double *x, const double *y,
+                const double *z, const double *m, const double *range2, int nkernel,
+                const double *wx, const double *wy, const double *wz, const double *u,
+                double step, double *out, int nthreads)
+{
+    if (n < 1 || nks < 0 || !check_kernel(nkernel) || !(step > 0.0)) return -1;
+    for (int64_t r = 0; r < nks; ++r)
+        if (ks[r] < 0 || ks[r] >= n) return -1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int64_t r = 0; r < nks; ++r) {
+        const int k = (int)ks[r];
+        double v[5] = {x[k], y[k], z[k], m[k], 0.0};
+        const double H = decode_H(range2[k]), sgn = range2[k] > 0.0 ? 1.0 : -1.0;
+        for (int c = 0; c < 5; ++c) {
+            double lp, lm, h;
+            if (c < 4) {
+                h = step * fmax(1.0, fabs(v[c]));
+                double p[4] = {v[0], v[1], v[2], v[3]}, q[4] = {v[0], v[1], v[2], v[3]};
+                p[c] += h;
+                q[c] -= h;
+                lp = loss_particle_at(n, k, p[0], p[1], p[2], p[3], range2[k], x, y, z, m, range2,
+                                      nkernel, wx, wy, wz, u);
+                lm = loss_particle_at(n, k, q[0], q[1], q[2], q[3], range2[k], x, y, z, m, range2,
+                                      nkernel, wx, wy, wz, u);
+            } else {
+                h = step;
+                lp = loss_particle_at(n, k, v[0], v[1], v[2], v[3], sgn * (H + h) * (H + h), x, y,
+                                      z, m, range2, nkernel, wx, wy, wz, u);
+                lm = loss_particle_at(n, k, v[0], v[1], v[2], v[3], sgn * (H - h) * (H - h), x, y,
+                                      z, m, range2, nkernel, wx, wy, wz, u);
+            }
+            out[5 * r + c] = (lp - lm) / (2.0 * h);
+        }
     }
-    return L;
+    return 0;
 }
 
 /* Total-energy diagnostic epot = 1/2 sum m_i phi_i (output.f:232-252). */

@@ -47,6 +47,12 @@ struct sg_context {
     unsigned long long *stats = nullptr;   // [2]
     double *in[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double *out[4] = {nullptr, nullptr, nullptr, nullptr};
+    // adjoint: sorted cotangents; host-path staging of cotangents and gradients
+    double *scot[4] = {nullptr, nullptr, nullptr, nullptr};
+    int64_t cap_cot = 0;
+    double *vin[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *vout[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int64_t cap_vin = 0, cap_vout = 0;
 
     // tile shape per precision (autotuned on the N = 2^20 giant-star set)
     int wg[2] = {256, 256};
@@ -270,7 +276,10 @@ int sg_destroy(sg_context *c)
                     c->pb.im, c->pb.iH, c->pb.iidx, c->sort_tmp, c->partial, c->nearflag, c->stats,
                     c->in[0], c->in[1], c->in[2], c->in[3], c->in[4],
                     c->out[0], c->out[1], c->out[2], c->out[3],
-                    c->d_offsets, c->d_sets, c->d_items, c->d_counter};
+                    c->d_offsets, c->d_sets, c->d_items, c->d_counter,
+                    c->scot[0], c->scot[1], c->scot[2], c->scot[3],
+                    c->vin[0], c->vin[1], c->vin[2], c->vin[3],
+                    c->vout[0], c->vout[1], c->vout[2], c->vout[3], c->vout[4]};
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -632,6 +641,126 @@ int sg_compute_host(sg_context *c, int n, int ibeg, int ni, const double *x, con
     double *dst[4] = {ax, ay, az, pot};
     for (int k = 0; k < 4 && ni > 0; ++k)
         HIP_TRY(hipMemcpy(dst[k], c->out[k], sizeof(double) * (size_t)ni, hipMemcpyDeviceToHost));
+    return SG_OK;
+}
+
+// ------------------------------------------------------------ adjoint
+int sg_vjp_device(sg_context *c, int n, const double *x, const double *y, const double *z,
+                  const double *m, const double *range2, int nkernel, int precision,
+                  const double *wx, const double *wy, const double *wz, const double *u,
+                  double *gx, double *gy, double *gz, double *gm, double *gH)
+{
+    if (!c) return SG_ERR_ARG;
+    if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
+    if (precision != SG_PREC_FP32C && precision != SG_PREC_FP64) return SG_ERR_ARG;
+    if (n < 1) return SG_ERR_ARG;
+    if (!x || !y || !z || !m || !range2 || !wx || !wy || !wz || !u) return SG_ERR_ARG;
+    if (!gx || !gy || !gz || !gm || !gH) return SG_ERR_ARG;
+    HIP_TRY(hipSetDevice(c->device));
+    c->launched = false;
+
+    const int64_t padded = ((int64_t)(n) + TJ - 1) / TJ * TJ;
+    int rc = ensure_capacity(c, n, padded, 1);
+    if (rc != SG_OK) return rc;
+    for (int k = 0; k < 4; ++k) {
+        int64_t cap = c->cap_cot;
+        if ((rc = grow(&c->scot[k], &cap, padded)) != SG_OK) return rc;
+    }
+    c->cap_cot = std::max<int64_t>(c->cap_cot, padded);
+    hipStream_t s = c->stream;
+    sg::PrepBuffers &b = c->pb;
+    HIP_TRY(hipEventRecord(c->ev_p0, s));
+    HIP_TRY(sg::launch_validate_bbox(x, y, z, m, range2, n, b, s));
+    HIP_TRY(sg::launch_morton(x, y, z, 0, n, b, s));
+    HIP_TRY(sg::sort_pairs(c->sort_tmp, c->sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
+                           b.vals_out, n, s));
+    HIP_TRY(sg::launch_gather(x, y, z, m, range2, b.vals_out, n, padded, b.sx, b.sy, b.sz, b.sm,
+                              b.sH, b.sidx, s));
+    const int ntiles = (int)(padded / TJ);
+    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, ntiles, b.tiles, s));
+    HIP_TRY(sg::launch_gather_cot(wx, wy, wz, u, b.vals_out, n, padded, c->scot[0], c->scot[1],
+                                  c->scot[2], c->scot[3], b, s));
+
+    sg::AdjArgs a{};
+    a.p = sg::SortedSet{b.sx, b.sy, b.sz, b.sm, b.sH, b.sidx, n, padded};
+    a.wx = c->scot[0];
+    a.wy = c->scot[1];
+    a.wz = c->scot[2];
+    a.u = c->scot[3];
+    a.tiles = b.tiles;
+    a.n = n;
+    a.nkernel = nkernel;
+    a.ntiles = ntiles;
+    const int wg = sg::adj_config_supported(c->wg[precision]) ? c->wg[precision] : 256;
+    a.niblocks = (int)((n + (int64_t)wg - 1) / wg);
+    a.ngroups = (int64_t)a.niblocks * (wg / 64);
+    a.jsplit = auto_jsplit(a.niblocks, wg, ntiles);
+    a.gx = gx;
+    a.gy = gy;
+    a.gz = gz;
+    a.gm = gm;
+    a.gH = gH;
+    if ((rc = grow(&c->partial, &c->cap_part, (int64_t)a.jsplit * 5 * n)) != SG_OK) return rc;
+    a.partial = c->partial;
+    if ((rc = grow(&c->nearflag, &c->cap_flag, a.ngroups * ntiles)) != SG_OK) return rc;
+    a.nearflag = c->nearflag;
+    a.stats = c->stats;
+    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 4, s));
+
+    HIP_TRY(hipEventRecord(c->ev_k0, s));
+    HIP_TRY(sg::launch_adj_far(precision, a, wg, s));
+    HIP_TRY(hipEventRecord(c->ev_kf, s));
+    HIP_TRY(sg::launch_adj_near(precision, a, s));
+    HIP_TRY(sg::launch_adj_reduce(a, s));
+    HIP_TRY(hipEventRecord(c->ev_k1, s));
+    c->launched = true;
+    return SG_OK;
+}
+
+int sg_vjp_host(sg_context *c, int n, const double *x, const double *y, const double *z,
+                const double *m, const double *range2, int nkernel, int precision,
+                const double *wx, const double *wy, const double *wz, const double *u,
+                double *gx, double *gy, double *gz, double *gm, double *gH)
+{
+    if (!c) return SG_ERR_ARG;
+    if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
+    if (n < 1) return SG_ERR_ARG;
+    if (!x || !y || !z || !m || !range2 || !wx || !wy || !wz || !u) return SG_ERR_ARG;
+    if (!gx || !gy || !gz || !gm || !gH) return SG_ERR_ARG;
+    HIP_TRY(hipSetDevice(c->device));
+    int rc;
+    for (int k = 0; k < 5; ++k) {
+        int64_t cap = c->cap_in;
+        if ((rc = grow(&c->in[k], &cap, n)) != SG_OK) return rc;
+    }
+    c->cap_in = std::max<int64_t>(c->cap_in, n);
+    for (int k = 0; k < 4; ++k) {
+        int64_t cap = c->cap_vin;
+        if ((rc = grow(&c->vin[k], &cap, n)) != SG_OK) return rc;
+    }
+    c->cap_vin = std::max<int64_t>(c->cap_vin, n);
+    for (int k = 0; k < 5; ++k) {
+        int64_t cap = c->cap_vout;
+        if ((rc = grow(&c->vout[k], &cap, n)) != SG_OK) return rc;
+    }
+    c->cap_vout = std::max<int64_t>(c->cap_vout, n);
+    const double *src[5] = {x, y, z, m, range2};
+    for (int k = 0; k < 5; ++k)
+        HIP_TRY(hipMemcpyAsync(c->in[k], src[k], sizeof(double) * (size_t)n, hipMemcpyHostToDevice,
+                               c->stream));
+    const double *csrc[4] = {wx, wy, wz, u};
+    for (int k = 0; k < 4; ++k)
+        HIP_TRY(hipMemcpyAsync(c->vin[k], csrc[k], sizeof(double) * (size_t)n,
+                               hipMemcpyHostToDevice, c->stream));
+    rc = sg_vjp_device(c, n, c->in[0], c->in[1], c->in[2], c->in[3], c->in[4], nkernel, precision,
+                       c->vin[0], c->vin[1], c->vin[2], c->vin[3], c->vout[0], c->vout[1],
+                       c->vout[2], c->vout[3], c->vout[4]);
+    if (rc != SG_OK) return rc;
+    rc = sg_synchronize(c);
+    if (rc != SG_OK) return rc;
+    double *dst[5] = {gx, gy, gz, gm, gH};
+    for (int k = 0; k < 5; ++k)
+        HIP_TRY(hipMemcpy(dst[k], c->vout[k], sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
     return SG_OK;
 }
 

@@ -1,0 +1,114 @@
+// sg_device.h -- device helpers shared by the pair kernels (sg_direct.hip)
+// and the adjoint kernels (sg_adjoint.hip): wave-uniform values and
+// reductions, the tile box test, the kernel polynomials, FP64 rsqrt.
+#pragma once
+
+#include "sg_internal.h"
+
+#include <math.h>
+
+namespace sg {
+
+constexpr float FAR_MARGIN = 1.0e-3f;   // relative guard on the box-distance test
+// Hi-only differences x_j,hi - x_i,hi carry a relative error of about
+// 2^-24 (|x_i - O| + |x_j - O| + r) / r <= 2^-24 (2 + 2 |x_i - O| / r), with
+// |x_i - O| <= sqrt(3) reach (reach = per-axis max over the wave).  Tiles
+// nearer than sqrt(3) WIDE_K reach use hi+lo differences, which bounds the
+// hi-only ones by 2^-24 (2 + 2 / WIDE_K) = 6 ulp (3.6e-7).  Waves that
+// straddle a jump of the space-filling curve have a large reach; with
+// WIDE_K = 0.5 about 2% of tiles take the hi+lo body on the giant-star set.
+constexpr float WIDE_K = 0.5f;
+
+__device__ __forceinline__ float uniform_f(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return uniform_f(v);
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return uniform_f(v);
+}
+
+// Squared distance between a wave's i-box [bl, bh] and tile tm's box, both
+// relative to the FP64 origin O.
+__device__ __forceinline__ float box_dist2(const TileMeta &tm, double Ox, double Oy, double Oz,
+                                           const float (&bl)[3], const float (&bh)[3])
+{
+    const float lx = (float)(tm.lo[0] - Ox), ly = (float)(tm.lo[1] - Oy), lz = (float)(tm.lo[2] - Oz);
+    const float ux = (float)(tm.hi[0] - Ox), uy = (float)(tm.hi[1] - Oy), uz = (float)(tm.hi[2] - Oz);
+    const float gx = fmaxf(0.0f, fmaxf(lx - bh[0], bl[0] - ux));
+    const float gy = fmaxf(0.0f, fmaxf(ly - bh[1], bl[1] - uy));
+    const float gz = fmaxf(0.0f, fmaxf(lz - bh[2], bl[2] - uz));
+    return fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+}
+
+// True when every pair of the wave and the tile is at least the larger
+// support radius apart, i.e. every pair takes the Newtonian branch.
+__device__ __forceinline__ bool boxes_far(float d2, float hmax, const TileMeta &tm)
+{
+    const float hm = fmaxf(hmax, (float)tm.maxH);
+    return d2 > hm * hm * (1.0f + FAR_MARGIN);
+}
+
+// Kernel polynomials A(q) (force) and P(q) (potential), q = r/H, with the
+// reference's decimal coefficients (cpu_grav.f:91-117, 210-243, 321-350).
+__device__ __forceinline__ void poly_AP(int nk, double q, double invq, double &A, double &P)
+{
+    const double q2 = q * q;
+    if (nk == 0) {
+        if (q < 0.5) {
+            A = 10.666666666667 + q2 * (32.0 * q - 38.4);
+            P = -2.8 + q2 * (5.333333333333 + q2 * (6.4 * q - 9.6));
+        } else {
+            const double iq3 = invq * invq * invq;
+            A = 21.333333333333 + q * (-48.0 + q * (38.4 - 10.666666666667 * q))
+                - 0.066666666667 * iq3;
+            P = -3.2 + 0.066666666667 * invq
+                + q2 * (10.666666666667 + q * (-16.0 + q * (9.6 - 2.133333333333 * q)));
+        }
+    } else if (nk == 1) {
+        A = 28.4375 + q2 * (-187.6875 + q2 * (804.375 + q2 * (-4379.375 + q * (9009.0
+            + q * (-8957.8125 + q * (5005.0 + q * (-1515.9375 + 195.0 * q)))))));
+        P = -3.828125 + q2 * (14.21875 + q2 * (-46.921875 + q2 * (134.0625 + q2 * (
+            -547.421875 + q * (1001.0 + q * (-895.78125 + q * (455.0 + q * (
+            -126.328125 + 15.0 * q))))))));
+    } else {
+        A = 20.625 + q2 * (-115.5 + q2 * (618.75 + q * (-1155.0 + q * (962.5
+            + q * (-396.0 + 65.625 * q)))));
+        P = -3.4375 + q2 * (10.3125 + q2 * (-28.875 + q2 * (103.125 + q * (-165.0
+            + q * (120.3125 + q * (-44.0 + 6.5625 * q))))));
+    }
+}
+
+__device__ __forceinline__ double self_coeff(int nk)
+{
+    return nk == 0 ? 1.4 : (nk == 1 ? 1.9140625 : 1.71875);
+}
+
+// 1/sqrt(x) in FP64: FAST takes the FP32 hardware estimate (1 ulp) and one
+// Newton step, y' = y + y (1 - x y^2) / 2, leaving ~1e-14 relative error;
+// otherwise the FP64 library rsqrt.
+template <bool FAST>
+__device__ __forceinline__ double rsqrt64(double x)
+{
+    if (!FAST || !(x > 1.0e-36 && x < 1.0e36)) return rsqrt(x);   // outside FP32 range
+    const double y = (double)__builtin_amdgcn_rsqf((float)x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// Orders the LDS writes of one wave before its reads (and the reverse).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+}  // namespace sg

@@ -35,6 +35,7 @@
 // i-blocks x j-partitions), and a batch of independent sets in one persistent
 // launch whose workgroups pull (set, i-block) items from a queue.
 #include "sg_internal.h"
+#include "sg_device.h"
 
 #include <math.h>
 
@@ -42,59 +43,11 @@ namespace sg {
 namespace {
 
 constexpr int TS = 128;                 // j per LDS stage; TJ / TS stages per tile
-constexpr float FAR_MARGIN = 1.0e-3f;   // relative guard on the box-distance test
-// Hi-only differences x_j,hi - x_i,hi carry a relative error of about
-// 2^-24 (|x_i - O| + |x_j - O| + r) / r <= 2^-24 (2 + 2 |x_i - O| / r), with
-// |x_i - O| <= sqrt(3) reach (reach = per-axis max over the wave).  Tiles
-// nearer than sqrt(3) WIDE_K reach use hi+lo differences, which bounds the
-// hi-only ones by 2^-24 (2 + 2 / WIDE_K) = 6 ulp (3.6e-7).  Waves that
-// straddle a jump of the space-filling curve have a large reach; with
-// WIDE_K = 0.5 about 2% of tiles take the hi+lo body on the giant-star set.
-constexpr float WIDE_K = 0.5f;
 constexpr int FOLD = 32;                // FP32 partial sums span FOLD j
 constexpr int NEAR_WAVES = 4;           // waves sharing one group's near tiles
 
 static_assert(TJ % TS == 0, "tile must be a whole number of stages");
 static_assert(TS % FOLD == 0, "stage must be a whole number of folds");
-
-// ---------------------------------------------------------------- helpers
-__device__ __forceinline__ float uniform_f(float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
-}
-__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-__device__ __forceinline__ float wave_min(float v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return uniform_f(v);
-}
-__device__ __forceinline__ float wave_max(float v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return uniform_f(v);
-}
-
-// Squared distance between a wave's i-box [bl, bh] and tile tm's box, both
-// relative to the FP64 origin O.
-__device__ __forceinline__ float box_dist2(const TileMeta &tm, double Ox, double Oy, double Oz,
-                                           const float (&bl)[3], const float (&bh)[3])
-{
-    const float lx = (float)(tm.lo[0] - Ox), ly = (float)(tm.lo[1] - Oy), lz = (float)(tm.lo[2] - Oz);
-    const float ux = (float)(tm.hi[0] - Ox), uy = (float)(tm.hi[1] - Oy), uz = (float)(tm.hi[2] - Oz);
-    const float gx = fmaxf(0.0f, fmaxf(lx - bh[0], bl[0] - ux));
-    const float gy = fmaxf(0.0f, fmaxf(ly - bh[1], bl[1] - uy));
-    const float gz = fmaxf(0.0f, fmaxf(lz - bh[2], bl[2] - uz));
-    return fmaf(gx, gx, fmaf(gy, gy, gz * gz));
-}
-
-// True when every pair of the wave and the tile is at least the larger
-// support radius apart, i.e. every pair takes the Newtonian branch.
-__device__ __forceinline__ bool boxes_far(float d2, float hmax, const TileMeta &tm)
-{
-    const float hm = fmaxf(hmax, (float)tm.maxH);
-    return d2 > hm * hm * (1.0f + FAR_MARGIN);
-}
 
 // The part of a launch one far-kernel workgroup works on.
 struct Block {
@@ -354,49 +307,16 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
     }
 }
 
-// ------------------------------------------------- softened pair, FP64
-// Kernel polynomials A(q) (force) and P(q) (potential), q = r/H, with the
-// reference's decimal coefficients (cpu_grav.f:91-117, 210-243, 321-350).
-__device__ __forceinline__ void poly_AP(int nk, double q, double invq, double &A, double &P)
-{
-    const double q2 = q * q;
-    if (nk == 0) {
-        if (q < 0.5) {
-            A = 10.666666666667 + q2 * (32.0 * q - 38.4);
-            P = -2.8 + q2 * (5.333333333333 + q2 * (6.4 * q - 9.6));
-        } else {
-            const double iq3 = invq * invq * invq;
-            A = 21.333333333333 + q * (-48.0 + q * (38.4 - 10.666666666667 * q))
-                - 0.066666666667 * iq3;
-            P = -3.2 + 0.066666666667 * invq
-                + q2 * (10.666666666667 + q * (-16.0 + q * (9.6 - 2.133333333333 * q)));
-        }
-    } else if (nk == 1) {
-        A = 28.4375 + q2 * (-187.6875 + q2 * (804.375 + q2 * (-4379.375 + q * (9009.0
-            + q * (-8957.8125 + q * (5005.0 + q * (-1515.9375 + 195.0 * q)))))));
-        P = -3.828125 + q2 * (14.21875 + q2 * (-46.921875 + q2 * (134.0625 + q2 * (
-            -547.421875 + q * (1001.0 + q * (-895.78125 + q * (455.0 + q * (
-            -126.328125 + 15.0 * q))))))));
-    } else {
-        A = 20.625 + q2 * (-115.5 + q2 * (618.75 + q * (-1155.0 + q * (962.5
-            + q * (-396.0 + 65.625 * q)))));
-        P = -3.4375 + q2 * (10.3125 + q2 * (-28.875 + q2 * (103.125 + q * (-165.0
-            + q * (120.3125 + q * (-44.0 + 6.5625 * q))))));
-    }
-}
-
-__device__ __forceinline__ double self_coeff(int nk)
-{
-    return nk == 0 ? 1.4 : (nk == 1 ? 1.9140625 : 1.71875);
-}
-
 // One side k of a softened pair: adds F(r,H_k) and P(r,H_k) (without m_j).
-__device__ __forceinline__ void soft_side(int nk, double r, double ri, double r2, double H,
+// The kernel choice NK is a template parameter of the near kernels: with one
+// polynomial per instantiation the register footprint drops by a third.
+template <int NK>
+__device__ __forceinline__ void soft_side(double r, double ri, double r2, double H,
                                           double invH, double &gacc, double &gpot)
 {
     if (r2 <= H * H) {
         double A, P;
-        poly_AP(nk, r * invH, H * ri, A, P);
+        poly_AP(NK, r * invH, H * ri, A, P);
         gacc += A * invH * invH * invH;
         gpot += P * invH;
     } else {
@@ -405,22 +325,10 @@ __device__ __forceinline__ void soft_side(int nk, double r, double ri, double r2
     }
 }
 
-// 1/sqrt(x) in FP64: FAST takes the FP32 hardware estimate (1 ulp) and one
-// Newton step, y' = y + y (1 - x y^2) / 2, leaving ~1e-14 relative error;
-// otherwise the FP64 library rsqrt.
-template <bool FAST>
-__device__ __forceinline__ double rsqrt64(double x)
-{
-    if (!FAST || !(x > 1.0e-36 && x < 1.0e36)) return rsqrt(x);   // outside FP32 range
-    const double y = (double)__builtin_amdgcn_rsqf((float)x);
-    const double e = fma(-x * y, y, 1.0);
-    return fma(0.5 * y, e, y);
-}
-
 // Ordered pair (i <- j), FP64, the semantics of grav_force_direct.cu:55-396
 // with the r == 0 rule documented in include/sphgrav.h.
-template <bool FAST>
-__device__ __forceinline__ void near_pair(int nk, double dx, double dy, double dz, double Hi,
+template <bool FAST, int NK>
+__device__ __forceinline__ void near_pair(double dx, double dy, double dz, double Hi,
                                           double invHi, double Hj, double mj, bool gasj,
                                           bool same, double &ax, double &ay, double &az,
                                           double &ap)
@@ -436,13 +344,13 @@ __device__ __forceinline__ void near_pair(int nk, double dx, double dy, double d
         az = fma(mr3, dz, az);
         ap -= mr;
     } else if (r2 == 0.0) {
-        if (!same || gasj) ap -= self_coeff(nk) * mj * (invHi + 1.0 / Hj);
+        if (!same || gasj) ap -= self_coeff(NK) * mj * (invHi + 1.0 / Hj);
     } else {
         const double ri = rsqrt64<FAST>(r2);
         const double r = r2 * ri;
         double gacc = 0.0, gpot = 0.0;
-        soft_side(nk, r, ri, r2, Hi, invHi, gacc, gpot);
-        soft_side(nk, r, ri, r2, Hj, 1.0 / Hj, gacc, gpot);
+        soft_side<NK>(r, ri, r2, Hi, invHi, gacc, gpot);
+        soft_side<NK>(r, ri, r2, Hj, 1.0 / Hj, gacc, gpot);
         const double hm = 0.5 * mj;
         const double f = hm * gacc;
         ax = fma(f, dx, ax);
@@ -454,15 +362,10 @@ __device__ __forceinline__ void near_pair(int nk, double dx, double dy, double d
 
 // ------------------------------------------------------------ near group
 // One workgroup of NEAR_WAVES waves per i-group.  The waves deal the group's
-// flagged tiles round-robin among themselves, stage each tile in their own
-// LDS slice, and at the end combine their sums in a fixed order (wave
-// 0 + 1 + 2 + 3) before adding them to the far kernel's partials.
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
+// flagged tiles round-robin among themselves, stage each tile in 64-particle
+// chunks in their own LDS slice, and at the end combine their sums in a
+// fixed order (wave 0 + 1 + 2 + 3) before adding them to the far kernel's
+// partials.
 __device__ __forceinline__ bool group_has_near(const Group &g)
 {
     for (int t = g.t0; t < g.t1; ++t)
@@ -470,13 +373,17 @@ __device__ __forceinline__ bool group_has_near(const Group &g)
     return false;
 }
 
+constexpr int NEAR_CHUNK = 64;   // j per LDS chunk of a near wave (one per lane)
+static_assert(TJ % NEAR_CHUNK == 0, "tile must be a whole number of chunks");
+
+// 12 KB per workgroup, so LDS does not cap occupancy below the register limit
 struct NearLds {
-    double4 p[NEAR_WAVES][TJ];   // x, y, z relative to the origin; m
-    double2 h[NEAR_WAVES][TJ];   // signed H, original index (as double)
+    double4 p[NEAR_WAVES][NEAR_CHUNK];   // x, y, z relative to the origin; m
+    double2 h[NEAR_WAVES][NEAR_CHUNK];   // signed H, original index (as double)
 };
 
-template <int IPL, bool FAST>
-__device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &J, int nk,
+template <int IPL, bool FAST, int NK>
+__device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &J,
                                            const Group &g, NearLds &lds, double *s_red)
 {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -506,23 +413,22 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
     for (int t = g.t0; t < g.t1; ++t) {
         if (!g.flags[t - g.t0]) continue;
         if ((cnt++ % NEAR_WAVES) != w) continue;
-        const int64_t j0 = (int64_t)t * TJ;
-        wave_lds_sync();
-        for (int q = 0; q < TJ / 64; ++q) {
-            const int jj = q * 64 + lane;
-            const int64_t j = j0 + jj;
-            lds.p[w][jj] = make_double4(J.x[j] - Ox, J.y[j] - Oy, J.z[j] - Oz, J.m[j]);
-            lds.h[w][jj] = make_double2(J.Hs[j], (double)J.idx[j]);
-        }
-        wave_lds_sync();
-        for (int jj = 0; jj < TJ; ++jj) {
-            const double4 p = lds.p[w][jj];
-            const double2 h = lds.h[w][jj];
+        for (int c0 = 0; c0 < TJ; c0 += NEAR_CHUNK) {
+            const int64_t j = (int64_t)t * TJ + c0 + lane;
+            wave_lds_sync();
+            lds.p[w][lane] = make_double4(J.x[j] - Ox, J.y[j] - Oy, J.z[j] - Oz, J.m[j]);
+            lds.h[w][lane] = make_double2(J.Hs[j], (double)J.idx[j]);
+            wave_lds_sync();
+#pragma unroll 1
+            for (int jj = 0; jj < NEAR_CHUNK; ++jj) {
+                const double4 p = lds.p[w][jj];
+                const double2 h = lds.h[w][jj];
 #pragma unroll
-            for (int k = 0; k < IPL; ++k)
-                near_pair<FAST>(nk, p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k], invHi[k],
-                                fabs(h.x), p.w, h.x > 0.0, (int)h.y == idi[k], ax[k], ay[k],
-                                az[k], ap[k]);
+                for (int k = 0; k < IPL; ++k)
+                    near_pair<FAST, NK>(p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k], invHi[k],
+                                    fabs(h.x), p.w, h.x > 0.0, (int)h.y == idi[k], ax[k], ay[k],
+                                    az[k], ap[k]);
+            }
         }
     }
 
@@ -594,7 +500,7 @@ __global__ __launch_bounds__(WG) void k_far_fp64(PairArgs a)
     far_block_fp64<WG, IPL>(a.i, a.j, a.tiles, single_block<WG, IPL>(a), a.stats, s_p);
 }
 
-template <int IPL, bool FAST>
+template <int IPL, bool FAST, int NK>
 __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near(PairArgs a)
 {
     __shared__ NearLds lds;
@@ -609,7 +515,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near(PairArgs a)
     g.part = a.partial + (int64_t)jp * 4 * a.ni;
     g.pstride = a.ni;
     if (g.gbase >= g.iend || !group_has_near(g)) return;
-    near_group<IPL, FAST>(a.i, a.j, a.nkernel, g, lds, s_red);
+    near_group<IPL, FAST, NK>(a.i, a.j, g, lds, s_red);
 }
 
 // Fixed-order FP64 sum of the j-partition partials, scattered to input order.
@@ -672,7 +578,7 @@ __global__ __launch_bounds__(WG) void k_far_batch(BatchArgs a)
     }
 }
 
-template <int IPL, bool FAST>
+template <int IPL, bool FAST, int NK>
 __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_batch(BatchArgs a)
 {
     __shared__ NearLds lds;
@@ -696,7 +602,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_batch(BatchArgs a)
     g.part = a.partial;
     g.pstride = a.padded;
     if (g.gbase >= g.iend || !group_has_near(g)) return;
-    near_group<IPL, FAST>(a.j, a.j, a.nkernel, g, lds, s_red);
+    near_group<IPL, FAST, NK>(a.j, a.j, g, lds, s_red);
 }
 
 // Scatter the per-slot sums of a batch back to input order.
@@ -737,21 +643,32 @@ hipError_t launch_pair_far(int precision, const PairArgs &a, int wg, hipStream_t
     return hipErrorInvalidConfiguration;
 }
 
+template <int IPL, bool FAST>
+static void near_launch(const PairArgs &a, dim3 grid, dim3 block, hipStream_t s)
+{
+    switch (a.nkernel) {
+    case 0: hipLaunchKernelGGL((k_near<IPL, FAST, 0>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_near<IPL, FAST, 1>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_near<IPL, FAST, 2>), grid, block, 0, s, a); break;
+    }
+}
+
 hipError_t launch_pair_near(int precision, const PairArgs &a, hipStream_t s)
 {
     if (a.ni <= 0) return hipSuccess;
+    if (a.nkernel < 0 || a.nkernel > 2) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(a.ngroups * a.jsplit)), block(64 * NEAR_WAVES);
     if (precision == 0) {
         switch (a.ipl) {
-        case 1: hipLaunchKernelGGL((k_near<1, true>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_near<2, true>), grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_near<4, true>), grid, block, 0, s, a); break;
+        case 1: near_launch<1, true>(a, grid, block, s); break;
+        case 2: near_launch<2, true>(a, grid, block, s); break;
+        case 4: near_launch<4, true>(a, grid, block, s); break;
         default: return hipErrorInvalidConfiguration;
         }
     } else {
         switch (a.ipl) {
-        case 1: hipLaunchKernelGGL((k_near<1, false>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_near<2, false>), grid, block, 0, s, a); break;
+        case 1: near_launch<1, false>(a, grid, block, s); break;
+        case 2: near_launch<2, false>(a, grid, block, s); break;
         default: return hipErrorInvalidConfiguration;
         }
     }
@@ -819,8 +736,19 @@ bool batch_config_supported(int wg, int ipl)
     return (wg == 64 || wg == 128 || wg == 256) && (ipl == 1 || ipl == 2);
 }
 
+template <int IPL, bool FAST>
+static void near_batch_launch(const BatchArgs &a, dim3 grid, dim3 block, hipStream_t s)
+{
+    switch (a.nkernel) {
+    case 0: hipLaunchKernelGGL((k_near_batch<IPL, FAST, 0>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_near_batch<IPL, FAST, 1>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_near_batch<IPL, FAST, 2>), grid, block, 0, s, a); break;
+    }
+}
+
 hipError_t launch_batch(int precision, const BatchArgs &a, int wg, int ipl, int nwg, hipStream_t s)
 {
+    if (a.nkernel < 0 || a.nkernel > 2) return hipErrorInvalidValue;
     if (a.padded <= 0) return hipSuccess;
     hipError_t e = launch_fill_zero_int((int *)a.counter, 1, s);
     if (e != hipSuccess) return e;
@@ -829,11 +757,11 @@ hipError_t launch_batch(int precision, const BatchArgs &a, int wg, int ipl, int 
     const dim3 block(64 * NEAR_WAVES), grid((unsigned)a.ngroups);
     if (a.ngroups > 0) {
         if (precision == 0) {
-            if (ipl == 1) hipLaunchKernelGGL((k_near_batch<1, true>), grid, block, 0, s, a);
-            else hipLaunchKernelGGL((k_near_batch<2, true>), grid, block, 0, s, a);
+            if (ipl == 1) near_batch_launch<1, true>(a, grid, block, s);
+            else near_batch_launch<2, true>(a, grid, block, s);
         } else {
-            if (ipl == 1) hipLaunchKernelGGL((k_near_batch<1, false>), grid, block, 0, s, a);
-            else hipLaunchKernelGGL((k_near_batch<2, false>), grid, block, 0, s, a);
+            if (ipl == 1) near_batch_launch<1, false>(a, grid, block, s);
+            else near_batch_launch<2, false>(a, grid, block, s);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }

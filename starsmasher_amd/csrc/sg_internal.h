@@ -80,6 +80,31 @@ struct BatchArgs {
     unsigned long long *stats;
 };
 
+// Vector-Jacobian product of one full evaluation (all n particles as i):
+// L = sum_i w_i . a_i + u_i phi_i, gradients w.r.t. x, y, z, m and the
+// support radius H.  Same tiles, origin and near flags as the forward pass.
+struct AdjArgs {
+    SortedSet p;                        // sorted particles (i and j)
+    const double *wx, *wy, *wz, *u;     // sorted-order cotangents (0 on padding)
+    const TileMeta *tiles;
+    int64_t n;
+    int nkernel;
+    int ntiles;
+    int jsplit;
+    int niblocks;                       // i-blocks of the far kernel
+    int64_t ngroups;                    // waves of the far kernel (64 i each)
+    double *gx, *gy, *gz, *gm, *gH;     // outputs, input order
+    double *partial;                    // [jsplit][5][n] sorted order
+    unsigned char *nearflag;            // [ngroups][ntiles]
+    unsigned long long *stats;          // [2]
+};
+
+// Adjoint launchers (sg_adjoint.hip): far, near, reduce in stream order.
+hipError_t launch_adj_far(int precision, const AdjArgs &a, int wg, hipStream_t s);
+hipError_t launch_adj_near(int precision, const AdjArgs &a, hipStream_t s);
+hipError_t launch_adj_reduce(const AdjArgs &a, hipStream_t s);
+bool adj_config_supported(int wg);
+
 // Launchers (sg_direct.hip).  Return hipError_t of the launch.
 // An evaluation is far kernel -> near kernel -> reduce, in stream order.
 hipError_t launch_pair_far(int precision, const PairArgs &a, int wg, hipStream_t s);
@@ -121,6 +146,12 @@ hipError_t launch_gather(const double *x, const double *y, const double *z,
                          const double *m, const double *r2, const int *perm, int count,
                          int64_t padded, double *ox, double *oy, double *oz, double *om,
                          double *oH, int *oidx, hipStream_t s);
+// Cotangents gathered into sorted order (0 on padding); non-finite values
+// set ERR_NONFINITE in b.err.
+hipError_t launch_gather_cot(const double *wx, const double *wy, const double *wz,
+                             const double *u, const int *perm, int count, int64_t padded,
+                             double *owx, double *owy, double *owz, double *ou,
+                             const PrepBuffers &b, hipStream_t s);
 hipError_t launch_tile_meta(const double *sx, const double *sy, const double *sz,
                             const double *sH, int ntiles, TileMeta *tiles, hipStream_t s);
 hipError_t launch_fill_zero_int(int *p, int n, hipStream_t s);

@@ -155,6 +155,33 @@ __global__ __launch_bounds__(RB) void k_gather(const double *__restrict__ x,
     oidx[k] = real ? src : -1;
 }
 
+// Cotangents of the adjoint into sorted order; padding slots get 0 so they
+// add nothing as j.  Any non-finite value sets ERR_NONFINITE.
+__global__ __launch_bounds__(RB) void k_gather_cot(const double *__restrict__ wx,
+                                                   const double *__restrict__ wy,
+                                                   const double *__restrict__ wz,
+                                                   const double *__restrict__ u,
+                                                   const int *__restrict__ perm, int count,
+                                                   int64_t padded, double *__restrict__ owx,
+                                                   double *__restrict__ owy,
+                                                   double *__restrict__ owz,
+                                                   double *__restrict__ ou, int *err)
+{
+    const int64_t k = blockIdx.x * (int64_t)RB + threadIdx.x;
+    if (k >= padded) return;
+    if (k >= count) {
+        owx[k] = owy[k] = owz[k] = ou[k] = 0.0;
+        return;
+    }
+    const int src = perm[k];
+    const double a = wx[src], b = wy[src], c = wz[src], d = u[src];
+    owx[k] = a;
+    owy[k] = b;
+    owz[k] = c;
+    ou[k] = d;
+    if (!(isfinite(a) && isfinite(b) && isfinite(c) && isfinite(d))) atomicOr(err, (int)ERR_NONFINITE);
+}
+
 __global__ __launch_bounds__(TJ) void k_tile_meta(const double *__restrict__ sx,
                                                   const double *__restrict__ sy,
                                                   const double *__restrict__ sz,
@@ -311,6 +338,17 @@ hipError_t launch_gather(const double *x, const double *y, const double *z, cons
     if (padded <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather, dim3(nblocks_for(padded)), dim3(RB), 0, s, x, y, z, m, r2, perm,
                        count, padded, ox, oy, oz, om, oH, oidx);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_cot(const double *wx, const double *wy, const double *wz,
+                             const double *u, const int *perm, int count, int64_t padded,
+                             double *owx, double *owy, double *owz, double *ou,
+                             const PrepBuffers &b, hipStream_t s)
+{
+    if (padded <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_cot, dim3(nblocks_for(padded)), dim3(RB), 0, s, wx, wy, wz, u,
+                       perm, count, padded, owx, owy, owz, ou, b.err);
     return hipGetLastError();
 }
 

@@ -183,6 +183,34 @@ class Gravity:
         check(st, "sg_compute_host")
         return tuple(outs)
 
+    def vjp(self, x, y, z, m, range2, wx, wy, wz, u, nkernel=0, precision=SG_PREC_FP32C):
+        """Blocking host-array adjoint: the gradient of
+        L = sum_i w_i . a_i + u_i phi_i over all particles with respect to
+        (x, y, z, m, H), H = sqrt|range2|.  Returns (gx, gy, gz, gm, gH)."""
+        n = len(x)
+        ins = [_host_f64(a, nm, n) for a, nm in ((x, "x"), (y, "y"), (z, "z"), (m, "m"),
+                                                  (range2, "range2"), (wx, "wx"), (wy, "wy"),
+                                                  (wz, "wz"), (u, "u"))]
+        outs = [np.empty(n) for _ in range(5)]
+        st = self._L.sg_vjp_host(self._h, n, *[_ptr(a) for a in ins[:5]], int(nkernel),
+                                 int(precision), *[_ptr(a) for a in ins[5:]],
+                                 *[_ptr(o) for o in outs])
+        check(st, "sg_vjp_host")
+        return tuple(outs)
+
+    def vjp_device(self, n, x, y, z, m, range2, wx, wy, wz, u, gx, gy, gz, gm, gH, nkernel=0,
+                   precision=SG_PREC_FP32C):
+        """Enqueue a device-resident adjoint (addresses or device arrays of
+        length n); call ``synchronize``."""
+        n = int(n)
+        ptrs = [_dev_ptr(a, nm, n) for a, nm in ((x, "x"), (y, "y"), (z, "z"), (m, "m"),
+                                                  (range2, "range2"), (wx, "wx"), (wy, "wy"),
+                                                  (wz, "wz"), (u, "u"), (gx, "gx"), (gy, "gy"),
+                                                  (gz, "gz"), (gm, "gm"), (gH, "gH"))]
+        vp = [C.c_void_p(p) for p in ptrs]
+        st = self._L.sg_vjp_device(self._h, n, *vp[:5], int(nkernel), int(precision), *vp[5:])
+        check(st, "sg_vjp_device")
+
     def compute_device(self, n, x, y, z, m, range2, ax, ay, az, pot, nkernel=0,
                        precision=SG_PREC_FP32C, ibeg=0, ni=None):
         """Enqueue a device-resident evaluation (torch tensors or addresses)."""

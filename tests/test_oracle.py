@@ -253,6 +253,44 @@ def test_vjp_matches_finite_differences(nk):
         assert abs(fd - gH[k]) < 1e-6 * max(1.0, abs(gH[k])), (k, fd, gH[k])
 
 
+@pytest.mark.parametrize("nk", [0, 1, 2])
+def test_vjp_coincident_pair_finite_differences(nk):
+    """Two distinct particles at the same position: the acceleration of each
+    is m F(r) dr with F finite at r = 0, so the x-gradient keeps F(0) terms."""
+    n = 24
+    pos, m, hp, u, r2 = _cloud(n, seed=50 + nk, npoint=1, hscale=(0.4, 1.2))
+    pos[5] = pos[4]
+    r = _rng(60 + nk)
+    w = r.normal(size=(n, 3))
+    cu = r.normal(size=n)
+    x, y, z = pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy()
+    gx, gy, gz, gm, gH = oracle.vjp(x, y, z, m, r2, w, cu, nkernel=nk)
+    eps = 1e-6
+    for k in (4, 5):
+        for arr, g in ((x, gx), (y, gy), (z, gz), (m, gm)):
+            keep = arr[k]
+            h = eps * max(1.0, abs(keep))
+            arr[k] = keep + h; lp = oracle.loss_particle(k, x, y, z, m, r2, w, cu, nkernel=nk)
+            arr[k] = keep - h; lm = oracle.loss_particle(k, x, y, z, m, r2, w, cu, nkernel=nk)
+            arr[k] = keep
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[k]) < 1e-6 * max(1.0, abs(g[k])), (k, fd, g[k])
+
+
+def test_fd_grad_agrees_with_vjp():
+    """The threaded central-difference helper used by the GPU adjoint tests
+    reproduces the adjoint on a small set (and so sums the same loss)."""
+    n = 200
+    pos, m, hp, u, r2 = _cloud(n, seed=70, npoint=3, hscale=(0.3, 0.9))
+    r = _rng(71)
+    w = r.normal(size=(n, 3))
+    cu = r.normal(size=n)
+    g = np.stack(oracle.vjp(pos[:, 0], pos[:, 1], pos[:, 2], m, r2, w, cu), 1)
+    ks = np.array([0, 3, 50, 199])
+    fd = oracle.fd_grad(ks, pos[:, 0], pos[:, 1], pos[:, 2], m, r2, w, cu, step=1e-6)
+    assert np.abs(fd - g[ks]).max() < 1e-7 * np.abs(g).max()
+
+
 def _dphidh_reference(nk, u):
     """kernels.f:215-279: dphidh (cubic), dphi332hdh, dphic42hdh, u = r/hp."""
     if nk == 0:
