@@ -168,6 +168,28 @@ int sg_last_timings(sg_context *ctx, float *prep_ms, float *far_ms, float *near_
  * how many took the softened (near-field) path. */
 int sg_last_tile_stats(sg_context *ctx, long long *visits, long long *near_visits);
 
+/* Evaluation that also forms the grad-h gravitational softening term of
+ * each gas particle i of the i-range (range2_i > 0; 0 for point particles):
+ *     psi_i = hp_i^-2 sum_{j: r_ij < 2 hp_i} m_j dphidh(r_ij / hp_i)
+ * over every j of the set, j = i included (dphidh(0) = c), with dphidh the
+ * closed forms of kernels.f:215-279 for the nkernel in use and hp = H/2.
+ * Replaces the neighbour sum bonet1_psi of advance.f90:716-737 (with
+ * nselfgravity = 1), which the reference evaluates from a 10^7-entry table
+ * on the CPU; here it is exact and comes out of the near-field pass that
+ * already visits every pair with r < H_i.  psi (length ni, input order)
+ * is written with ax..pot; other semantics as sg_compute_device / _host
+ * (the pass runs the near kernel with one i per lane). */
+int sg_compute_psi_device(sg_context *ctx, int n, int ibeg, int ni,
+                          const double *x, const double *y, const double *z,
+                          const double *m, const double *range2,
+                          int nkernel, int precision,
+                          double *ax, double *ay, double *az, double *pot, double *psi);
+int sg_compute_psi_host(sg_context *ctx, int n, int ibeg, int ni,
+                        const double *x, const double *y, const double *z,
+                        const double *m, const double *range2,
+                        int nkernel, int precision,
+                        double *ax, double *ay, double *az, double *pot, double *psi);
+
 /* Adjoint (no reference counterpart; the reference has no derivative of the
  * gravity step beyond kernels.f:215-279's dphi/dh, which the H-gradient here
  * generalises).  For the forward outputs over all n particles (ibeg = 0,

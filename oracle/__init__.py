@@ -91,6 +91,9 @@ def lib():
         L.sgo_fd_grad.argtypes = [C.c_int, _i64p, C.c_int64, _dp, _dp, _dp, _dp, _dp, C.c_int,
                                   _dp, _dp, _dp, _dp, C.c_double, _dp, C.c_int]
         L.sgo_fd_grad.restype = C.c_int
+        L.sgo_psi_rows.argtypes = [C.c_int, _i64p, C.c_int64, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                                   C.c_int, _dp, C.c_int]
+        L.sgo_psi_rows.restype = C.c_int
         L.sgo_kernel_AP.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.sgo_kernel_dAP.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.sgo_epot.argtypes = [C.c_int64, _dp, _dp]
@@ -229,6 +232,21 @@ def fd_grad(ks, x, y, z, m, range2, w, u, nkernel=0, step=1e-6, nthreads=0):
                            float(step), out.reshape(-1), int(nthreads or os.cpu_count() or 1))
     if rc != 0:
         raise ValueError("sgo_fd_grad rejected its arguments")
+    return out
+
+
+def psi(x, y, z, m, range2, nkernel=0, rows=None, tabulated=False, nthreads=0):
+    """Grad-h softening term psi_i = hp_i^-2 sum_j m_j dphidh(r_ij/hp_i) over
+    r_ij < 2 hp_i (advance.f90:716-737), for gas particles; 0 for point
+    particles.  tabulated=True reproduces the reference's 10^7-entry table."""
+    x, y, z, m, range2 = map(_f64, (x, y, z, m, range2))
+    n = x.size
+    rows = np.arange(n, dtype=np.int64) if rows is None else np.ascontiguousarray(rows, dtype=np.int64)
+    out = np.empty(rows.size)
+    rc = lib().sgo_psi_rows(n, rows, rows.size, x, y, z, m, range2, int(nkernel), int(bool(tabulated)),
+                            out, int(nthreads or os.cpu_count() or 1))
+    if rc != 0:
+        raise ValueError("sgo_psi_rows rejected its arguments")
     return out
 
 

@@ -13,7 +13,8 @@
  *   pbe/src/SPHgrav_lib2/grav_force_direct.cu:599-609  range2 decode: H=sqrt|range2|, flag=range2>0
  *   pbe/src/cpu_grav.f:1-381                           symmetric j>=i CPU twin
  *   pbe/src/main.f:297-349                             interaction-balanced i split
- *   pbe/src/kernels.f:215-279                          dphi/dh closed forms (adjoint check)
+ *   pbe/src/kernels.f:215-279                          dphi/dh closed forms (adjoint check, psi)
+ *   pbe/src/advance.f90:716-737                        grad-h softening sum psi_i
  *
  * Pinning (what checks each function; see tests/test_oracle.py):
  *   sgo_direct        golden grpot of the reference's own start dumps (cubic),
@@ -29,6 +30,9 @@
  *   sgo_fd_grad       the same central differences; its loss is pinned
  *                     through sgo_direct (the row/column sums it adds).
  *   sgo_partition     hand-worked splits and the invariants of main.f:297-349.
+ *   sgo_psi_rows      kernels.f's dphidh closed forms (tests restate them and
+ *                     tie them to P and A: dphidh = -(P(q) + q P'(q))/2), the
+ *                     self term dphidh(0) = c, and the table lookup's bound.
  */
 #include <math.h>
 #include <stdint.h>
@@ -724,6 +728,66 @@ int sgo_fd_grad(int n, const int64_t *ks, int64_t nks, const double *x, const do
             }
             out[5 * r + c] = (lp - lm) / (2.0 * h);
         }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Grad-h softening term psi_i (advance.f90:716-737, kernels.f:215-279):     */
+/*   psi_i = hp_i^-2 sum_{j: r_ij^2 < 4 hp_i^2} m_j dphidh(r_ij / hp_i)      */
+/* over every j (nselfgravity = 1), including j = i, for gas particles i      */
+/* (range2 > 0); 0 for point particles.  tabulated = 1 reproduces the         */
+/* reference's table lookup: dphidhtab(itab), itab = int(ctab r^2/hp^2) + 1,  */
+/* ctab = (ntab - 1)/4, entry k holds dphidh(sqrt(4 (k-1)/(ntab-1))) with     */
+/* ntab = 10^7 (starsmasher.h:18, kernels.f:20-54); tabulated = 0 evaluates   */
+/* dphidh exactly.                                                            */
+/* ------------------------------------------------------------------------ */
+static double dphidh_closed(int nkernel, double u)
+{
+    if (nkernel == 0) {                       /* kernels.f:215-257 */
+        if (u <= 1.0) return -2.0 * u * u + 1.5 * pow(u, 4) - 0.6 * pow(u, 5) + 1.4;
+        if (u < 2.0) return (0.1 + 0.2 * u) * pow(2.0 - u, 4);
+        return 0.0;
+    }
+    const double q = u / 2.0;
+    if (q >= 1.0) return 0.0;
+    if (nkernel == 1)                         /* kernels.f:259-268 */
+        return 35.0 * pow(1.0 - q, 9) * (7.0 + 63.0 * q + 237.0 * q * q + 453.0 * pow(q, 3)
+                                         + 384.0 * pow(q, 4)) / 128.0;
+    return 55.0 * pow(1.0 - q, 7) * (1.0 + 7.0 * q + 19.0 * q * q + 21.0 * pow(q, 3)) / 32.0;
+}
+
+int sgo_psi_rows(int n, const int64_t *rows, int64_t nrows, const double *x, const double *y,
+                 const double *z, const double *m, const double *range2, int nkernel,
+                 int tabulated, double *psi, int nthreads)
+{
+    if (n < 0 || nrows < 0 || !check_kernel(nkernel)) return -1;
+    for (int64_t r = 0; r < nrows; ++r)
+        if (rows[r] < 0 || rows[r] >= n) return -1;
+    const double ntab = 10000000.0, ctab = (ntab - 1.0) / 4.0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t r = 0; r < nrows; ++r) {
+        const int64_t i = rows[r];
+        if (!(range2[i] > 0.0)) { psi[r] = 0.0; continue; }
+        const double hp = 0.5 * decode_H(range2[i]), h2 = hp * hp;
+        double s = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            const double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (!(d2 < 4.0 * h2)) continue;
+            double v;
+            if (tabulated) {
+                const double itab = floor(ctab * d2 / h2) + 1.0;
+                v = dphidh_closed(nkernel, sqrt(4.0 * (itab - 1.0) / (ntab - 1.0)));
+            } else {
+                v = dphidh_closed(nkernel, sqrt(d2 / h2));
+            }
+            s += m[j] * v;
+        }
+        psi[r] = s / h2;
     }
     return 0;
 }

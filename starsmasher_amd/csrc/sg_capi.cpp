@@ -42,6 +42,8 @@ struct sg_context {
     sg::PrepBuffers pb{};
     void *sort_tmp = nullptr;
     double *partial = nullptr;
+    double *psipart = nullptr;          // grad-h term partial sums
+    int64_t cap_psipart = 0;
     unsigned char *nearflag = nullptr;
     int64_t cap_flag = 0;
     unsigned long long *stats = nullptr;   // [2]
@@ -53,6 +55,8 @@ struct sg_context {
     double *vin[4] = {nullptr, nullptr, nullptr, nullptr};
     double *vout[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int64_t cap_vin = 0, cap_vout = 0;
+    double *psi_out = nullptr;          // host-path staging of the grad-h term
+    int64_t cap_psi_out = 0;
 
     // tile shape per precision (autotuned on the N = 2^20 giant-star set)
     int wg[2] = {256, 256};
@@ -279,7 +283,8 @@ int sg_destroy(sg_context *c)
                     c->d_offsets, c->d_sets, c->d_items, c->d_counter,
                     c->scot[0], c->scot[1], c->scot[2], c->scot[3],
                     c->vin[0], c->vin[1], c->vin[2], c->vin[3],
-                    c->vout[0], c->vout[1], c->vout[2], c->vout[3], c->vout[4]};
+                    c->vout[0], c->vout[1], c->vout[2], c->vout[3], c->vout[4], c->psipart,
+                    c->psi_out};
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -312,9 +317,13 @@ int sg_get_config(sg_context *c, int precision, int *wg, int *ipl, int *jsplit)
     return SG_OK;
 }
 
-int sg_compute_device(sg_context *c, int n, int ibeg, int ni, const double *x, const double *y,
-                      const double *z, const double *m, const double *range2, int nkernel,
-                      int precision, double *ax, double *ay, double *az, double *pot)
+namespace {
+
+// One evaluation; psi (may be null) also forms the grad-h softening term.
+int compute_device_impl(sg_context *c, int n, int ibeg, int ni, const double *x,
+                        const double *y, const double *z, const double *m,
+                        const double *range2, int nkernel, int precision, double *ax,
+                        double *ay, double *az, double *pot, double *psi)
 {
     if (!c) return SG_ERR_ARG;
     if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
@@ -363,7 +372,8 @@ int sg_compute_device(sg_context *c, int n, int ibeg, int ni, const double *x, c
     a.ibeg = ibeg;
     a.nkernel = nkernel;
     a.ntiles = ntiles;
-    const int wg = c->wg[precision], ipl = c->ipl[precision];
+    // the grad-h term is formed by the near kernel with one i per lane
+    const int wg = c->wg[precision], ipl = psi ? 1 : c->ipl[precision];
     a.ipl = ipl;
     a.niblocks = (int)((ni + (int64_t)wg * ipl - 1) / ((int64_t)wg * ipl));
     a.ngroups = (int64_t)a.niblocks * (wg / 64);
@@ -378,6 +388,12 @@ int sg_compute_device(sg_context *c, int n, int ibeg, int ni, const double *x, c
     if ((rc = grow(&c->nearflag, &c->cap_flag, a.ngroups * ntiles)) != SG_OK) return rc;
     a.nearflag = c->nearflag;
     a.stats = c->stats;
+    if (psi) {
+        if ((int64_t)a.jsplit * ni > INT32_MAX / 2) return SG_ERR_ARG;
+        if ((rc = grow(&c->psipart, &c->cap_psipart, (int64_t)a.jsplit * ni)) != SG_OK) return rc;
+        a.psi = psi;
+        a.psipart = c->psipart;
+    }
     HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 4, s));
 
     HIP_TRY(hipEventRecord(c->ev_k0, s));
@@ -388,6 +404,26 @@ int sg_compute_device(sg_context *c, int n, int ibeg, int ni, const double *x, c
     HIP_TRY(hipEventRecord(c->ev_k1, s));
     c->launched = true;
     return SG_OK;
+}
+
+}  // namespace
+
+int sg_compute_device(sg_context *c, int n, int ibeg, int ni, const double *x, const double *y,
+                      const double *z, const double *m, const double *range2, int nkernel,
+                      int precision, double *ax, double *ay, double *az, double *pot)
+{
+    return compute_device_impl(c, n, ibeg, ni, x, y, z, m, range2, nkernel, precision, ax, ay, az,
+                               pot, nullptr);
+}
+
+int sg_compute_psi_device(sg_context *c, int n, int ibeg, int ni, const double *x,
+                          const double *y, const double *z, const double *m,
+                          const double *range2, int nkernel, int precision, double *ax,
+                          double *ay, double *az, double *pot, double *psi)
+{
+    if (ni > 0 && !psi) return SG_ERR_ARG;
+    return compute_device_impl(c, n, ibeg, ni, x, y, z, m, range2, nkernel, precision, ax, ay, az,
+                               pot, psi);
 }
 
 // ----------------------------------------------------- batch of sets
@@ -608,9 +644,12 @@ int sg_last_tile_stats(sg_context *c, long long *visits, long long *near_visits)
     return SG_OK;
 }
 
-int sg_compute_host(sg_context *c, int n, int ibeg, int ni, const double *x, const double *y,
-                    const double *z, const double *m, const double *range2, int nkernel,
-                    int precision, double *ax, double *ay, double *az, double *pot)
+namespace {
+
+int compute_host_impl(sg_context *c, int n, int ibeg, int ni, const double *x, const double *y,
+                      const double *z, const double *m, const double *range2, int nkernel,
+                      int precision, double *ax, double *ay, double *az, double *pot,
+                      double *psi)
 {
     if (!c) return SG_ERR_ARG;
     if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
@@ -628,21 +667,47 @@ int sg_compute_host(sg_context *c, int n, int ibeg, int ni, const double *x, con
         int64_t cap = c->cap_out;
         if ((rc = grow(&c->out[k], &cap, std::max(ni, 1))) != SG_OK) return rc;
     }
+    if (psi && (rc = grow(&c->psi_out, &c->cap_psi_out, std::max(ni, 1))) != SG_OK) return rc;
     c->cap_out = std::max<int64_t>(c->cap_out, std::max(ni, 1));
     const double *src[5] = {x, y, z, m, range2};
     for (int k = 0; k < 5; ++k)
         HIP_TRY(hipMemcpyAsync(c->in[k], src[k], sizeof(double) * (size_t)n, hipMemcpyHostToDevice,
                                c->stream));
-    rc = sg_compute_device(c, n, ibeg, ni, c->in[0], c->in[1], c->in[2], c->in[3], c->in[4],
-                           nkernel, precision, c->out[0], c->out[1], c->out[2], c->out[3]);
+    rc = compute_device_impl(c, n, ibeg, ni, c->in[0], c->in[1], c->in[2], c->in[3], c->in[4],
+                             nkernel, precision, c->out[0], c->out[1], c->out[2], c->out[3],
+                             psi ? c->psi_out : nullptr);
     if (rc != SG_OK) return rc;
     rc = sg_synchronize(c);
     if (rc != SG_OK) return rc;
     double *dst[4] = {ax, ay, az, pot};
     for (int k = 0; k < 4 && ni > 0; ++k)
         HIP_TRY(hipMemcpy(dst[k], c->out[k], sizeof(double) * (size_t)ni, hipMemcpyDeviceToHost));
+    if (psi && ni > 0)
+        HIP_TRY(hipMemcpy(psi, c->psi_out, sizeof(double) * (size_t)ni, hipMemcpyDeviceToHost));
     return SG_OK;
 }
+
+}  // namespace
+
+int sg_compute_host(sg_context *c, int n, int ibeg, int ni, const double *x, const double *y,
+                    const double *z, const double *m, const double *range2, int nkernel,
+                    int precision, double *ax, double *ay, double *az, double *pot)
+{
+    return compute_host_impl(c, n, ibeg, ni, x, y, z, m, range2, nkernel, precision, ax, ay, az,
+                             pot, nullptr);
+}
+
+int sg_compute_psi_host(sg_context *c, int n, int ibeg, int ni, const double *x, const double *y,
+                        const double *z, const double *m, const double *range2, int nkernel,
+                        int precision, double *ax, double *ay, double *az, double *pot,
+                        double *psi)
+{
+    if (ni > 0 && !psi) return SG_ERR_ARG;
+    return compute_host_impl(c, n, ibeg, ni, x, y, z, m, range2, nkernel, precision, ax, ay, az,
+                             pot, psi);
+}
+
+
 
 // ------------------------------------------------------------ adjoint
 int sg_vjp_device(sg_context *c, int n, const double *x, const double *y, const double *z,

@@ -67,9 +67,42 @@ struct Group {
     const unsigned char *flags;   // [t - t0]
     double *part;
     int64_t pstride;
+    double *psipart;              // psi partial sums of this j-partition (PSI kernels)
 };
 
 // ------------------------------------------------ far block, FP32 + comp.
+// One Newtonian pair (i <- j) in FP32.  FIRST starts the partial sums (the
+// first j of a fold writes them, which saves clearing them).
+template <bool WIDE, bool FIRST>
+__device__ __forceinline__ void far_pair_fp32(const float4 &p, const float4 &q, float xh, float yh,
+                                              float zh, float xl, float yl, float zl, float &fx,
+                                              float &fy, float &fz, float &fp)
+{
+    float dx = p.x - xh, dy = p.y - yh, dz = p.z - zh;
+    if (WIDE) {
+        dx += q.x - xl;
+        dy += q.y - yl;
+        dz += q.z - zl;
+    }
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float ri = __builtin_amdgcn_rsqf(r2);
+    const float mr = p.w * ri;
+    const float mr3 = mr * ri * ri;
+    if (FIRST) {
+        fx = mr3 * dx;
+        fy = mr3 * dy;
+        fz = mr3 * dz;
+        fp = mr;
+    } else {
+        fx = fmaf(mr3, dx, fx);
+        fy = fmaf(mr3, dy, fy);
+        fz = fmaf(mr3, dz, fz);
+        fp += mr;
+    }
+}
+
+// One LDS stage of far pairs.  The FOLD j of a fold are unrolled, so the
+// LDS reads use immediate offsets from one address per fold.
 template <int IPL, bool WIDE>
 __device__ __forceinline__ void far_stage_fp32(const float4 *s_hi, const float4 *s_lo,
                                                const float (&xh)[IPL], const float (&yh)[IPL],
@@ -78,32 +111,27 @@ __device__ __forceinline__ void far_stage_fp32(const float4 *s_hi, const float4 
                                                double (&ax)[IPL], double (&ay)[IPL],
                                                double (&az)[IPL], double (&ap)[IPL])
 {
+#pragma unroll 1
     for (int j0 = 0; j0 < TS; j0 += FOLD) {
+        const float4 *ph = s_hi + j0;
+        const float4 *pl = s_lo + j0;
         float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
+        {
+            const float4 p = ph[0];
+            const float4 q = WIDE ? pl[0] : p;
 #pragma unroll
-        for (int k = 0; k < IPL; ++k) fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
-#pragma unroll 4
-        for (int jj = j0; jj < j0 + FOLD; ++jj) {
-            const float4 p = s_hi[jj];
-            float4 q;
-            if (WIDE) q = s_lo[jj];
+            for (int k = 0; k < IPL; ++k)
+                far_pair_fp32<WIDE, true>(p, q, xh[k], yh[k], zh[k], xl[k], yl[k], zl[k], fx[k],
+                                          fy[k], fz[k], fp[k]);
+        }
 #pragma unroll
-            for (int k = 0; k < IPL; ++k) {
-                float dx = p.x - xh[k], dy = p.y - yh[k], dz = p.z - zh[k];
-                if (WIDE) {
-                    dx += q.x - xl[k];
-                    dy += q.y - yl[k];
-                    dz += q.z - zl[k];
-                }
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const float ri = __builtin_amdgcn_rsqf(r2);
-                const float mr = p.w * ri;
-                const float mr3 = mr * ri * ri;
-                fx[k] = fmaf(mr3, dx, fx[k]);
-                fy[k] = fmaf(mr3, dy, fy[k]);
-                fz[k] = fmaf(mr3, dz, fz[k]);
-                fp[k] += mr;
-            }
+        for (int jj = 1; jj < FOLD; ++jj) {
+            const float4 p = ph[jj];
+            const float4 q = WIDE ? pl[jj] : p;
+#pragma unroll
+            for (int k = 0; k < IPL; ++k)
+                far_pair_fp32<WIDE, false>(p, q, xh[k], yh[k], zh[k], xl[k], yl[k], zl[k], fx[k],
+                                           fy[k], fz[k], fp[k]);
         }
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
@@ -310,15 +338,20 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
 // One side k of a softened pair: adds F(r,H_k) and P(r,H_k) (without m_j).
 // The kernel choice NK is a template parameter of the near kernels: with one
 // polynomial per instantiation the register footprint drops by a third.
-template <int NK>
+// With PSI it also adds P(q) + q^2 A(q) to *ps when the side is softened: the
+// grad-h term's dphidh(2q) = -(P(q) + q^2 A(q))/2 (kernels.f:215-279).
+template <int NK, bool PSI = false>
 __device__ __forceinline__ void soft_side(double r, double ri, double r2, double H,
-                                          double invH, double &gacc, double &gpot)
+                                          double invH, double &gacc, double &gpot,
+                                          double *ps = nullptr)
 {
     if (r2 <= H * H) {
+        const double q = r * invH;
         double A, P;
-        poly_AP(NK, r * invH, H * ri, A, P);
+        poly_AP(NK, q, H * ri, A, P);
         gacc += A * invH * invH * invH;
         gpot += P * invH;
+        if (PSI) *ps = fma(q * q, A, P);
     } else {
         gacc += ri * ri * ri;
         gpot -= ri;
@@ -327,11 +360,13 @@ __device__ __forceinline__ void soft_side(double r, double ri, double r2, double
 
 // Ordered pair (i <- j), FP64, the semantics of grav_force_direct.cu:55-396
 // with the r == 0 rule documented in include/sphgrav.h.
-template <bool FAST, int NK>
+// With PSI, ps accumulates m_j (P(q_i) + q_i^2 A(q_i)) over the pairs with
+// r <= H_i (the grad-h softening sum before its -2/H_i^2 factor).
+template <bool FAST, int NK, bool PSI = false>
 __device__ __forceinline__ void near_pair(double dx, double dy, double dz, double Hi,
                                           double invHi, double Hj, double mj, bool gasj,
                                           bool same, double &ax, double &ay, double &az,
-                                          double &ap)
+                                          double &ap, double *ps = nullptr)
 {
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const double hj2 = Hj * Hj;
@@ -345,12 +380,15 @@ __device__ __forceinline__ void near_pair(double dx, double dy, double dz, doubl
         ap -= mr;
     } else if (r2 == 0.0) {
         if (!same || gasj) ap -= self_coeff(NK) * mj * (invHi + 1.0 / Hj);
+        if (PSI) *ps -= 2.0 * self_coeff(NK) * mj;   // P(0) = -2c
     } else {
         const double ri = rsqrt64<FAST>(r2);
         const double r = r2 * ri;
         double gacc = 0.0, gpot = 0.0;
-        soft_side<NK>(r, ri, r2, Hi, invHi, gacc, gpot);
+        double psk = 0.0;
+        soft_side<NK, PSI>(r, ri, r2, Hi, invHi, gacc, gpot, &psk);
         soft_side<NK>(r, ri, r2, Hj, 1.0 / Hj, gacc, gpot);
+        if (PSI) *ps = fma(mj, psk, *ps);
         const double hm = 0.5 * mj;
         const double f = hm * gacc;
         ax = fma(f, dx, ax);
@@ -382,16 +420,21 @@ struct NearLds {
     double2 h[NEAR_WAVES][NEAR_CHUNK];   // signed H, original index (as double)
 };
 
-template <int IPL, bool FAST, int NK>
+// With PSI (IPL = 1 only) it also forms the grad-h softening term
+// psi_i = -(2/H_i^2) sum_j m_j (P(q_i) + q_i^2 A(q_i)) of gas particles i and
+// adds it to g.psipart.
+template <int IPL, bool FAST, int NK, bool PSI = false>
 __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &J,
                                            const Group &g, NearLds &lds, double *s_red)
 {
+    static_assert(!PSI || IPL == 1, "psi is formed with one i per lane");
+    constexpr int NC = PSI ? 5 : 4;   // sums per i: ax, ay, az, pot (, psi)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t io = min(g.gbase + 32 * IPL, g.iend - 1);
     const double Ox = I.x[io], Oy = I.y[io], Oz = I.z[io];
 
     double xr[IPL], yr[IPL], zr[IPL], Hi[IPL], invHi[IPL];
-    double ax[IPL], ay[IPL], az[IPL], ap[IPL];
+    double ax[IPL], ay[IPL], az[IPL], ap[IPL], ps[IPL];
     int idi[IPL];
     int64_t ii[IPL];
     bool valid[IPL];
@@ -406,7 +449,7 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
         Hi[k] = fabs(I.Hs[ii[k]]);
         invHi[k] = 1.0 / Hi[k];
         idi[k] = I.idx[ii[k]];
-        ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+        ax[k] = ay[k] = az[k] = ap[k] = ps[k] = 0.0;
     }
 
     int cnt = 0;
@@ -425,9 +468,10 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
                 const double2 h = lds.h[w][jj];
 #pragma unroll
                 for (int k = 0; k < IPL; ++k)
-                    near_pair<FAST, NK>(p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k], invHi[k],
-                                    fabs(h.x), p.w, h.x > 0.0, (int)h.y == idi[k], ax[k], ay[k],
-                                    az[k], ap[k]);
+                    near_pair<FAST, NK, PSI>(p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k],
+                                             invHi[k], fabs(h.x), p.w, h.x > 0.0,
+                                             (int)h.y == idi[k], ax[k], ay[k], az[k], ap[k],
+                                             &ps[k]);
             }
         }
     }
@@ -436,11 +480,12 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
     if (w > 0) {
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
-            double *r = s_red + (((w - 1) * 4) * IPL + k) * 64 + lane;
+            double *r = s_red + (((w - 1) * NC) * IPL + k) * 64 + lane;
             r[0] = ax[k];
             r[IPL * 64] = ay[k];
             r[2 * IPL * 64] = az[k];
             r[3 * IPL * 64] = ap[k];
+            if (PSI) r[4 * IPL * 64] = ps[k];
         }
     }
     __syncthreads();
@@ -448,17 +493,21 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
         for (int v = 0; v < NEAR_WAVES - 1; ++v) {
-            const double *r = s_red + ((v * 4) * IPL + k) * 64 + lane;
+            const double *r = s_red + ((v * NC) * IPL + k) * 64 + lane;
             ax[k] += r[0];
             ay[k] += r[IPL * 64];
             az[k] += r[2 * IPL * 64];
             ap[k] += r[3 * IPL * 64];
+            if (PSI) ps[k] += r[4 * IPL * 64];
         }
         if (!valid[k]) continue;
         g.part[ii[k]] += ax[k];
         g.part[g.pstride + ii[k]] += ay[k];
         g.part[2 * g.pstride + ii[k]] += az[k];
         g.part[3 * g.pstride + ii[k]] += ap[k];
+        // point particles (negative H) carry no psi (advance.f90:750-753)
+        if (PSI && I.Hs[ii[k]] > 0.0)
+            g.psipart[ii[k]] += -2.0 * invHi[k] * invHi[k] * ps[k];
     }
 }
 
@@ -518,6 +567,26 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near(PairArgs a)
     near_group<IPL, FAST, NK>(a.i, a.j, g, lds, s_red);
 }
 
+// Near kernel that also forms the grad-h softening term (one i per lane).
+template <bool FAST, int NK>
+__global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_psi(PairArgs a)
+{
+    __shared__ NearLds lds;
+    __shared__ double s_red[(NEAR_WAVES - 1) * 5 * 64];
+    const int64_t gi = blockIdx.x % a.ngroups;
+    const int jp = (int)(blockIdx.x / a.ngroups);
+    Group g;
+    g.gbase = gi * 64;
+    g.iend = a.ni;
+    tile_range(a, jp, g.t0, g.t1);
+    g.flags = a.nearflag + gi * a.ntiles + g.t0;
+    g.part = a.partial + (int64_t)jp * 4 * a.ni;
+    g.pstride = a.ni;
+    g.psipart = a.psipart + (int64_t)jp * a.ni;
+    if (g.gbase >= g.iend || !group_has_near(g)) return;
+    near_group<1, FAST, NK, true>(a.i, a.j, g, lds, s_red);
+}
+
 // Fixed-order FP64 sum of the j-partition partials, scattered to input order.
 __global__ __launch_bounds__(256) void k_reduce_partials(PairArgs a)
 {
@@ -533,6 +602,11 @@ __global__ __launch_bounds__(256) void k_reduce_partials(PairArgs a)
     a.ay[o] = s[1];
     a.az[o] = s[2];
     a.pot[o] = s[3];
+    if (a.psi) {
+        double p = 0.0;
+        for (int jp = 0; jp < a.jsplit; ++jp) p += a.psipart[(int64_t)jp * a.ni + i];
+        a.psi[o] = p;
+    }
 }
 
 // ============================================================= batch kernels
@@ -653,11 +727,30 @@ static void near_launch(const PairArgs &a, dim3 grid, dim3 block, hipStream_t s)
     }
 }
 
+template <bool FAST>
+static void near_psi_launch(const PairArgs &a, dim3 grid, dim3 block, hipStream_t s)
+{
+    switch (a.nkernel) {
+    case 0: hipLaunchKernelGGL((k_near_psi<FAST, 0>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_near_psi<FAST, 1>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_near_psi<FAST, 2>), grid, block, 0, s, a); break;
+    }
+}
+
 hipError_t launch_pair_near(int precision, const PairArgs &a, hipStream_t s)
 {
     if (a.ni <= 0) return hipSuccess;
     if (a.nkernel < 0 || a.nkernel > 2) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(a.ngroups * a.jsplit)), block(64 * NEAR_WAVES);
+    if (a.psi) {
+        if (a.ipl != 1) return hipErrorInvalidConfiguration;
+        hipError_t e = launch_fill_zero_int(reinterpret_cast<int *>(a.psipart),
+                                            (int)(2 * (int64_t)a.jsplit * a.ni), s);
+        if (e != hipSuccess) return e;
+        if (precision == 0) near_psi_launch<true>(a, grid, block, s);
+        else near_psi_launch<false>(a, grid, block, s);
+        return hipGetLastError();
+    }
     if (precision == 0) {
         switch (a.ipl) {
         case 1: near_launch<1, true>(a, grid, block, s); break;

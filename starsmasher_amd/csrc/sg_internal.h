@@ -50,6 +50,8 @@ struct PairArgs {
     double *partial;              // [jsplit][4][ni] sorted-order partial sums
     unsigned char *nearflag;      // [ngroups][ntiles]: 1 = tile takes the softened path
     unsigned long long *stats;    // [2]: tile visits, near visits (may be null)
+    double *psi;                  // grad-h softening term, input order (null: not formed)
+    double *psipart;              // [jsplit][ni] its partial sums (with psi)
 };
 
 // A batch of independent particle sets laid out back to back in one padded

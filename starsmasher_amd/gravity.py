@@ -183,6 +183,21 @@ class Gravity:
         check(st, "sg_compute_host")
         return tuple(outs)
 
+    def compute_psi(self, x, y, z, m, range2, nkernel=0, precision=SG_PREC_FP32C, ibeg=0,
+                    ni=None):
+        """Blocking host-array evaluation that also forms the grad-h softening
+        term psi (advance.f90:716-737).  Returns (ax, ay, az, pot, psi) of the
+        0-based i-range [ibeg, ibeg+ni)."""
+        n = len(x)
+        ni = n - ibeg if ni is None else int(ni)
+        ins = [_host_f64(a, nm, n) for a, nm in ((x, "x"), (y, "y"), (z, "z"), (m, "m"),
+                                                  (range2, "range2"))]
+        outs = [np.empty(max(ni, 0)) for _ in range(5)]
+        st = self._L.sg_compute_psi_host(self._h, n, int(ibeg), ni, *[_ptr(a) for a in ins],
+                                         int(nkernel), int(precision), *[_ptr(o) for o in outs])
+        check(st, "sg_compute_psi_host")
+        return tuple(outs)
+
     def vjp(self, x, y, z, m, range2, wx, wy, wz, u, nkernel=0, precision=SG_PREC_FP32C):
         """Blocking host-array adjoint: the gradient of
         L = sum_i w_i . a_i + u_i phi_i over all particles with respect to

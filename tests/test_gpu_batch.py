@@ -138,3 +138,25 @@ def test_batch_argument_errors(grav):
     assert call([0, 4, 4, 10]) == _lib.SG_ERR_ARG        # empty set
     assert call([1, 4, 10]) == _lib.SG_ERR_ARG           # offsets[0] != 0
     assert call([0, 6, 3]) == _lib.SG_ERR_ARG            # decreasing
+
+
+def test_batch64_full_config(grav):
+    """The 64-star relaxation batch at full size (2.6M particles, N per star
+    log-uniform in [2048, 131072]) in one graph-captured launch: every star
+    against the FP64 batch on all particles and against the oracle on 16
+    random rows plus its 4 worst FP32C rows."""
+    stars = ics.batch_set(64)
+    res32 = grav.compute_batch(stars, precision=SG_PREC_FP32C, use_graph=True)
+    res64 = grav.compute_batch(stars, precision=SG_PREC_FP64, use_graph=True)
+    rng = np.random.default_rng(0)
+    worst = 0.0
+    for k, (d, r32, r64) in enumerate(zip(stars, res32, res64)):
+        ea, ep = rel_errors(np.stack(r32[:3], -1), r32[3], np.stack(r64[:3], -1), r64[3])
+        worst = max(worst, ea.max())
+        assert ea.max() <= 1e-5 and ep.max() <= 1e-6, (k, ea.max(), ep.max())
+        n = d["x"].size
+        rows = np.unique(np.concatenate([rng.choice(n, 16, replace=False), np.argsort(ea)[-4:]]))
+        ref = oracle.direct(d["x"], d["y"], d["z"], d["m"], d["range2"], nkernel=0, rows=rows)
+        _check(f"batch64 star{k} n={n}", SG_PREC_FP32C, [x[rows] for x in r32], ref)
+        _check(f"batch64 star{k} n={n}", SG_PREC_FP64, [x[rows] for x in r64], ref)
+    print(f"[batch64] worst star acc max vs FP64 batch {worst:.3e}")

@@ -257,3 +257,55 @@ def test_two_polytropes_262k(grav, two_poly):
         a = np.stack(r[:3], -1)
         tot = np.linalg.norm((m[:, None] * a).sum(0))
         assert tot <= 1e-7 * (m * np.linalg.norm(a, axis=1)).sum()
+
+
+def test_giant_1m_full_size(grav):
+    """The 1,048,576-particle giant star with a 30% point mass at full size:
+    FP32C against the FP64 kernel on every particle, both against the oracle
+    on 256 random rows plus the 32 worst FP32C rows (the FP32C bound is the
+    max relative acceleration error 1e-5 against FP64)."""
+    d = ics.giant_with_core(1 << 20)
+    n = d["x"].size
+    r64 = grav.compute(*_args(d), nkernel=0, precision=SG_PREC_FP64)
+    r32 = grav.compute(*_args(d), nkernel=0, precision=SG_PREC_FP32C)
+    ea, ep = rel_errors(np.stack(r32[:3], -1), r32[3], np.stack(r64[:3], -1), r64[3])
+    print(f"[giant1m fp32c vs fp64 kernel] acc max {ea.max():.3e} med {np.median(ea):.3e} "
+          f"pot max {ep.max():.3e} med {np.median(ep):.3e}")
+    rng = np.random.default_rng(0)
+    rows = np.unique(np.concatenate([rng.choice(n, 256, replace=False), np.argsort(ea)[-32:],
+                                     np.argsort(ep)[-16:]]))
+    ref = oracle.direct(*_args(d), nkernel=0, rows=rows)
+    _check("giant1m rows", SG_PREC_FP64, [r[rows] for r in r64], ref)
+    _check("giant1m rows", SG_PREC_FP32C, [r[rows] for r in r32], ref)
+    assert ea.max() <= 1e-5 and ep.max() <= 1e-6
+
+
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+@pytest.mark.parametrize("nk", [0, 1, 2])
+def test_grad_h_psi(grav, plummer16k, nk, prec):
+    """psi_i = hp_i^-2 sum_j m_j dphidh(r_ij/hp_i) (advance.f90:716-737) from
+    the near-field pass, with point particles and a coincident pair, against
+    the oracle's exact sum and the reference's 10^7-entry table lookup; the
+    accelerations and potentials of the same call match the plain call's."""
+    d = {k: v.copy() for k, v in plummer16k.items()}
+    d["range2"][-5:] = -np.abs(d["range2"][-5:])      # point particles: psi = 0
+    for k in ("x", "y", "z"):
+        d[k][11] = d[k][10]                            # a distinct coincident pair
+    n = d["x"].size
+    ax, ay, az, pot, psi = grav.compute_psi(*_args(d), nkernel=nk, precision=prec)
+    plain = grav.compute(*_args(d), nkernel=nk, precision=prec)
+    for a, b in zip((ax, ay, az, pot), plain):
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=0)
+    rows = np.unique(np.concatenate([np.arange(0, n, 61), [10, 11, n - 1]]))
+    ref = oracle.psi(*_args(d), nkernel=nk, rows=rows)
+    tab = oracle.psi(*_args(d), nkernel=nk, rows=rows, tabulated=True)
+    gas = ref != 0.0
+    rel = np.abs(psi[rows][gas] - ref[gas]) / np.abs(ref[gas])
+    print(f"[psi nk={nk} prec={prec}] vs exact max {rel.max():.3e}; "
+          f"exact vs table max {np.max(np.abs(tab[gas] - ref[gas]) / np.abs(ref[gas])):.3e}")
+    assert rel.max() <= 1e-12
+    assert np.all(psi[rows][~gas] == 0.0) and np.all(psi[-5:] == 0.0)
+    assert np.max(np.abs(psi[rows][gas] - tab[gas]) / np.abs(tab[gas])) <= 1e-6
+    # an i-range of the same set
+    sub = grav.compute_psi(*_args(d), nkernel=nk, precision=prec, ibeg=3000, ni=5000)
+    np.testing.assert_allclose(sub[4], psi[3000:8000], rtol=1e-13, atol=0)

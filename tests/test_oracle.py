@@ -334,3 +334,39 @@ def test_partition_invariants(n, nr):
     work = np.array([sum(n - i + 1 for i in range(a, b + 1)) for a, b in zip(lo, hi)])
     assert work.sum() == n * (n + 1) // 2
     assert work[:-1].max() / work[:-1].min() < 1.2
+
+
+# ------------------------------------------------------- grad-h softening psi
+@pytest.mark.parametrize("nk", [0, 1, 2])
+def test_psi_matches_softened_potential(nk):
+    """psi_i = hp^-2 sum_j m_j dphidh(r/hp) (advance.f90:716-737) against the
+    same sum formed from the potential polynomial: dphidh(u) =
+    -(P(q) + q^2 A(q))/2 with q = u/2, P' = q A (kernels.f:215-279)."""
+    n = 150
+    pos, m, hp, u, r2 = _cloud(n, seed=80 + nk, npoint=4, hscale=(0.4, 1.2))
+    got = oracle.psi(pos[:, 0], pos[:, 1], pos[:, 2], m, r2, nkernel=nk)
+    for i in (0, 17, 90, n - 5):
+        d = np.linalg.norm(pos - pos[i], axis=1)
+        H = 2 * hp[i]
+        want = 0.0
+        for j in np.nonzero(d < H)[0]:
+            q = d[j] / H
+            A, P = oracle.kernel_AP(nk, q)
+            want += m[j] * (-(P + q * q * A) / 2)
+        want /= hp[i] ** 2
+        assert abs(got[i] - want) <= 1e-10 * abs(want), (i, got[i], want)
+    assert np.all(got[-4:] == 0.0)          # point particles carry no psi
+
+
+def test_psi_self_term_and_table():
+    # an isolated gas particle: psi = m dphidh(0) / hp^2 = m c / hp^2
+    x = np.array([0.0, 50.0]); m = np.array([2.0, 1.0]); r2 = np.array([4 * 0.3 ** 2, 4.0])
+    for nk, c in ((0, 1.4), (1, 1.9140625), (2, 1.71875)):
+        p = oracle.psi(x, x, x, m, r2, nkernel=nk)
+        assert abs(p[0] - 2.0 * c / 0.09) < 1e-12 * p[0]
+    # the 10^7-entry table of the reference differs from the closed form by
+    # less than its spacing in u^2 (4e-7) times the slope
+    pos, m, hp, u, r2 = _cloud(200, seed=91, hscale=(0.4, 1.2))
+    a = oracle.psi(pos[:, 0], pos[:, 1], pos[:, 2], m, r2, tabulated=False)
+    b = oracle.psi(pos[:, 0], pos[:, 1], pos[:, 2], m, r2, tabulated=True)
+    assert np.max(np.abs(a - b) / np.abs(a)) < 1e-6
