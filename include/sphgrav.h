@@ -164,8 +164,9 @@ int sg_get_batch_config(sg_context *ctx, int precision, int *wg, int *ipl);
  * Any pointer may be NULL. */
 int sg_last_timings(sg_context *ctx, float *prep_ms, float *far_ms, float *near_ms);
 
-/* Statistics of the last evaluation: number of (wave, j-tile) visits and
- * how many took the softened (near-field) path. */
+/* Statistics of the last evaluation: number of (wave, 64-particle j-chunk)
+ * visits and how many of them took the near-field kernel (a chunk whose box
+ * test cannot rule out softened pairs for the wave). */
 int sg_last_tile_stats(sg_context *ctx, long long *visits, long long *near_visits);
 
 /* Evaluation that also forms the grad-h gravitational softening term of
@@ -189,6 +190,18 @@ int sg_compute_psi_host(sg_context *ctx, int n, int ibeg, int ni,
                         const double *m, const double *range2,
                         int nkernel, int precision,
                         double *ax, double *ay, double *az, double *pot, double *psi);
+
+/* The grad-h term alone (no accelerations), FP64: psi over the i-range as
+ * in sg_compute_psi_device, from a pass that visits only the 64-particle
+ * j-chunks within the i-group's largest H.  Replaces the psi part of
+ * advance.f90:716-737 (bonetsumwithphi) at a fraction of a gravity
+ * evaluation's cost.  Device pointers, asynchronous. */
+int sg_psi_device(sg_context *ctx, int n, int ibeg, int ni,
+                  const double *x, const double *y, const double *z,
+                  const double *m, const double *range2, int nkernel, double *psi);
+int sg_psi_host(sg_context *ctx, int n, int ibeg, int ni,
+                const double *x, const double *y, const double *z,
+                const double *m, const double *range2, int nkernel, double *psi);
 
 /* Adjoint (no reference counterpart; the reference has no derivative of the
  * gravity step beyond kernels.f:215-279's dphi/dh, which the H-gradient here
@@ -227,6 +240,14 @@ int sg_firsthalf(int n, int n_lower, int my_length,
                  const double *x, const double *y, const double *z,
                  const double *m, const double *range2, int nkernel);
 int sg_lasthalf(double *ax, double *ay, double *az, double *pot);
+/* Blocking grad-h term of particles n_lower .. n_lower+my_length-1 (1-based)
+ * of the set 1..n on the legacy layer's device, written to
+ * psi[n_lower-1 .. n_lower+my_length-2] like lasthalf writes its outputs:
+ * the Fortran-callable form of sg_psi_host for the density step
+ * (advance.f90:716-737), called before firsthalf. */
+int sg_grav_psi(int n, int n_lower, int my_length,
+                const double *x, const double *y, const double *z,
+                const double *m, const double *range2, int nkernel, double *psi);
 
 #ifdef __cplusplus
 }

@@ -10,10 +10,12 @@ symbols; this package adds the Python mirror of that interface
 (:mod:`starsmasher_amd.ics`).
 """
 from . import _lib
-from .gravity import (SG_PREC_FP32C, SG_PREC_FP64, Gravity, SphgravError, device_count,
-                      firsthalf_grav_forces, gpu_init_dev, gravforce, lasthalf_grav_forces)
+from .gravity import (SG_PREC_FP32C, SG_PREC_FP64, Gravity, SphgravError, bonet_psi,
+                      device_count, firsthalf_grav_forces, gpu_init_dev, gravforce,
+                      lasthalf_grav_forces)
 
 __all__ = [
     "Gravity", "SphgravError", "SG_PREC_FP32C", "SG_PREC_FP64", "device_count",
-    "firsthalf_grav_forces", "lasthalf_grav_forces", "gpu_init_dev", "gravforce", "_lib",
+    "firsthalf_grav_forces", "lasthalf_grav_forces", "gpu_init_dev", "gravforce", "bonet_psi",
+    "_lib",
 ]

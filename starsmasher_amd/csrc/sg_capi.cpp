@@ -170,6 +170,9 @@ int ensure_capacity(sg_context *c, int64_t n, int64_t padded, int64_t ni)
         if ((rc = grow(&c->pb.sidx, &cap, padded)) != SG_OK) return rc;
         cap = 0;
         if ((rc = grow(&c->pb.tiles, &cap, std::max<int64_t>(padded / TJ, 1))) != SG_OK) return rc;
+        cap = 0;
+        if ((rc = grow(&c->pb.chunks, &cap, std::max<int64_t>(padded / sg::TC, 1))) != SG_OK)
+            return rc;
         c->cap_pad = padded;
     }
     if (ni > c->cap_ni || !c->pb.ix) {
@@ -276,7 +279,7 @@ int sg_destroy(sg_context *c)
     (void)hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->pb.red, c->pb.bbox, c->pb.err, c->pb.keys_in, c->pb.keys_out,
                     c->pb.vals_in, c->pb.vals_out, c->pb.sx, c->pb.sy, c->pb.sz, c->pb.sm,
-                    c->pb.sH, c->pb.sidx, c->pb.tiles, c->pb.ix, c->pb.iy, c->pb.iz,
+                    c->pb.sH, c->pb.sidx, c->pb.tiles, c->pb.chunks, c->pb.ix, c->pb.iy, c->pb.iz,
                     c->pb.im, c->pb.iH, c->pb.iidx, c->sort_tmp, c->partial, c->nearflag, c->stats,
                     c->in[0], c->in[1], c->in[2], c->in[3], c->in[4],
                     c->out[0], c->out[1], c->out[2], c->out[3],
@@ -352,7 +355,7 @@ int compute_device_impl(sg_context *c, int n, int ibeg, int ni, const double *x,
     HIP_TRY(sg::launch_gather(x, y, z, m, range2, b.vals_out, n, padded, b.sx, b.sy, b.sz, b.sm,
                               b.sH, b.sidx, s));
     const int ntiles = (int)(padded / TJ);
-    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, ntiles, b.tiles, s));
+    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, ntiles, b.tiles, b.chunks, s));
 
     sg::PairArgs a{};
     a.j = sg::SortedSet{b.sx, b.sy, b.sz, b.sm, b.sH, b.sidx, n, padded};
@@ -368,6 +371,7 @@ int compute_device_impl(sg_context *c, int n, int ibeg, int ni, const double *x,
         a.i = sg::SortedSet{b.ix, b.iy, b.iz, b.im, b.iH, b.iidx, ni, ni};
     }
     a.tiles = b.tiles;
+    a.chunks = b.chunks;
     a.ni = ni;
     a.ibeg = ibeg;
     a.nkernel = nkernel;
@@ -414,6 +418,91 @@ int sg_compute_device(sg_context *c, int n, int ibeg, int ni, const double *x, c
 {
     return compute_device_impl(c, n, ibeg, ni, x, y, z, m, range2, nkernel, precision, ax, ay, az,
                                pot, nullptr);
+}
+
+int sg_psi_device(sg_context *c, int n, int ibeg, int ni, const double *x, const double *y,
+                  const double *z, const double *m, const double *range2, int nkernel,
+                  double *psi)
+{
+    if (!c) return SG_ERR_ARG;
+    if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
+    if (n < 1 || ibeg < 0 || ni < 0 || (int64_t)ibeg + ni > n) return SG_ERR_ARG;
+    if (!x || !y || !z || !m || !range2) return SG_ERR_ARG;
+    if (ni > 0 && !psi) return SG_ERR_ARG;
+    HIP_TRY(hipSetDevice(c->device));
+    c->launched = false;
+    if (ni == 0) return SG_OK;
+
+    const int64_t padded = ((int64_t)(n) + TJ - 1) / TJ * TJ;
+    int rc = ensure_capacity(c, n, padded, ni);
+    if (rc != SG_OK) return rc;
+    hipStream_t s = c->stream;
+    sg::PrepBuffers &b = c->pb;
+    HIP_TRY(hipEventRecord(c->ev_p0, s));
+    HIP_TRY(sg::launch_validate_bbox(x, y, z, m, range2, n, b, s));
+    HIP_TRY(sg::launch_morton(x, y, z, 0, n, b, s));
+    HIP_TRY(sg::sort_pairs(c->sort_tmp, c->sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
+                           b.vals_out, n, s));
+    HIP_TRY(sg::launch_gather(x, y, z, m, range2, b.vals_out, n, padded, b.sx, b.sy, b.sz, b.sm,
+                              b.sH, b.sidx, s));
+    const int ntiles = (int)(padded / TJ);
+    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, ntiles, b.tiles, b.chunks, s));
+    sg::PairArgs a{};
+    a.j = sg::SortedSet{b.sx, b.sy, b.sz, b.sm, b.sH, b.sidx, n, padded};
+    if (ibeg == 0 && ni == n) {
+        a.i = a.j;
+    } else {
+        HIP_TRY(sg::launch_morton(x, y, z, ibeg, ni, b, s));
+        HIP_TRY(sg::sort_pairs(c->sort_tmp, c->sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
+                               b.vals_out, ni, s));
+        HIP_TRY(sg::launch_gather(x, y, z, m, range2, b.vals_out, ni, ni, b.ix, b.iy, b.iz, b.im,
+                                  b.iH, b.iidx, s));
+        a.i = sg::SortedSet{b.ix, b.iy, b.iz, b.im, b.iH, b.iidx, ni, ni};
+    }
+    a.tiles = b.tiles;
+    a.chunks = b.chunks;
+    a.ni = ni;
+    a.ibeg = ibeg;
+    a.nkernel = nkernel;
+    a.ntiles = ntiles;
+    a.ipl = 1;
+    a.jsplit = 1;
+    a.ngroups = (ni + 63) / 64;
+    a.psi = psi;
+    HIP_TRY(hipEventRecord(c->ev_k0, s));
+    HIP_TRY(hipEventRecord(c->ev_kf, s));
+    HIP_TRY(sg::launch_psi_only(a, s));
+    HIP_TRY(hipEventRecord(c->ev_k1, s));
+    c->launched = true;
+    return SG_OK;
+}
+
+int sg_psi_host(sg_context *c, int n, int ibeg, int ni, const double *x, const double *y,
+                const double *z, const double *m, const double *range2, int nkernel, double *psi)
+{
+    if (!c) return SG_ERR_ARG;
+    if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
+    if (n < 1 || ibeg < 0 || ni < 0 || (int64_t)ibeg + ni > n) return SG_ERR_ARG;
+    if (!x || !y || !z || !m || !range2 || (ni > 0 && !psi)) return SG_ERR_ARG;
+    HIP_TRY(hipSetDevice(c->device));
+    int rc;
+    for (int k = 0; k < 5; ++k) {
+        int64_t cap = c->cap_in;
+        if ((rc = grow(&c->in[k], &cap, n)) != SG_OK) return rc;
+    }
+    c->cap_in = std::max<int64_t>(c->cap_in, n);
+    if ((rc = grow(&c->psi_out, &c->cap_psi_out, std::max(ni, 1))) != SG_OK) return rc;
+    const double *src[5] = {x, y, z, m, range2};
+    for (int k = 0; k < 5; ++k)
+        HIP_TRY(hipMemcpyAsync(c->in[k], src[k], sizeof(double) * (size_t)n, hipMemcpyHostToDevice,
+                               c->stream));
+    rc = sg_psi_device(c, n, ibeg, ni, c->in[0], c->in[1], c->in[2], c->in[3], c->in[4], nkernel,
+                       c->psi_out);
+    if (rc != SG_OK) return rc;
+    if ((rc = sg_synchronize(c)) != SG_OK) return rc;
+    if (ni > 0)
+        HIP_TRY(hipMemcpy(psi, c->psi_out, sizeof(double) * (size_t)ni, hipMemcpyDeviceToHost));
+    return SG_OK;
 }
 
 int sg_compute_psi_device(sg_context *c, int n, int ibeg, int ni, const double *x,
@@ -532,11 +621,13 @@ int sg_compute_batch_device(sg_context *c, int nsets, const long long *offsets, 
                            b.vals_out, (int)ntot, s));
     HIP_TRY(sg::launch_gather_batch(x, y, z, m, range2, b.vals_out, c->d_offsets, c->d_sets, nsets,
                                     P.padded, b.sx, b.sy, b.sz, b.sm, b.sH, b.sidx, s));
-    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, (int)(P.padded / TJ), b.tiles, s));
+    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, (int)(P.padded / TJ), b.tiles, b.chunks,
+                                 s));
 
     sg::BatchArgs a{};
     a.j = sg::SortedSet{b.sx, b.sy, b.sz, b.sm, b.sH, b.sidx, ntot, P.padded};
     a.tiles = b.tiles;
+    a.chunks = b.chunks;
     a.sets = c->d_sets;
     a.nsets = nsets;
     a.items = c->d_items;
@@ -742,7 +833,7 @@ int sg_vjp_device(sg_context *c, int n, const double *x, const double *y, const 
     HIP_TRY(sg::launch_gather(x, y, z, m, range2, b.vals_out, n, padded, b.sx, b.sy, b.sz, b.sm,
                               b.sH, b.sidx, s));
     const int ntiles = (int)(padded / TJ);
-    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, ntiles, b.tiles, s));
+    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, ntiles, b.tiles, b.chunks, s));
     HIP_TRY(sg::launch_gather_cot(wx, wy, wz, u, b.vals_out, n, padded, c->scot[0], c->scot[1],
                                   c->scot[2], c->scot[3], b, s));
 
@@ -966,6 +1057,24 @@ int sg_firsthalf(int n, int n_lower, int my_length, const double *x, const doubl
     L.ibeg = n_lower - 1;
     L.ni = ni;
     return SG_OK;
+}
+
+int sg_grav_psi(int n, int n_lower, int my_length, const double *x, const double *y,
+                const double *z, const double *m, const double *range2, int nkernel, double *psi)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (nkernel < 0 || nkernel > 2) return SG_ERR_NKERNEL;
+    if (n < 1 || n_lower < 1 || my_length < 0 || (int64_t)n_lower - 1 + my_length > n)
+        return SG_ERR_ARG;
+    if (!x || !y || !z || !m || !range2 || !psi) return SG_ERR_ARG;
+    int rc = check_host_inputs(n, x, y, z, m, range2);
+    if (rc != SG_OK) return rc;
+    sg_context *c = nullptr;
+    if ((rc = legacy_context(&c)) != SG_OK) return rc;
+    // stream order keeps a pending firsthalf's inputs intact until its
+    // kernels have read them; the call then waits for both
+    return sg_psi_host(c, n, n_lower - 1, my_length, x, y, z, m, range2, nkernel,
+                       psi + (n_lower - 1));
 }
 
 int sg_lasthalf(double *ax, double *ay, double *az, double *pot)

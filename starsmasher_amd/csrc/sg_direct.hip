@@ -47,7 +47,7 @@ constexpr int FOLD = 32;                // FP32 partial sums span FOLD j
 constexpr int NEAR_WAVES = 4;           // waves sharing one group's near tiles
 
 static_assert(TJ % TS == 0, "tile must be a whole number of stages");
-static_assert(TS % FOLD == 0, "stage must be a whole number of folds");
+static_assert(TS % TC == 0 && TC % FOLD == 0, "stage = whole chunks = whole folds");
 
 // The part of a launch one far-kernel workgroup works on.
 struct Block {
@@ -101,10 +101,10 @@ __device__ __forceinline__ void far_pair_fp32(const float4 &p, const float4 &q, 
     }
 }
 
-// One LDS stage of far pairs.  The FOLD j of a fold are unrolled, so the
-// LDS reads use immediate offsets from one address per fold.
+// One chunk (TC j) of far pairs from LDS.  The FOLD j of a fold are
+// unrolled, so the LDS reads use immediate offsets from one address per fold.
 template <int IPL, bool WIDE>
-__device__ __forceinline__ void far_stage_fp32(const float4 *s_hi, const float4 *s_lo,
+__device__ __forceinline__ void far_chunk_fp32(const float4 *s_hi, const float4 *s_lo,
                                                const float (&xh)[IPL], const float (&yh)[IPL],
                                                const float (&zh)[IPL], const float (&xl)[IPL],
                                                const float (&yl)[IPL], const float (&zl)[IPL],
@@ -112,7 +112,7 @@ __device__ __forceinline__ void far_stage_fp32(const float4 *s_hi, const float4 
                                                double (&az)[IPL], double (&ap)[IPL])
 {
 #pragma unroll 1
-    for (int j0 = 0; j0 < TS; j0 += FOLD) {
+    for (int j0 = 0; j0 < TC; j0 += FOLD) {
         const float4 *ph = s_hi + j0;
         const float4 *pl = s_lo + j0;
         float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
@@ -143,11 +143,29 @@ __device__ __forceinline__ void far_stage_fp32(const float4 *s_hi, const float4 
     }
 }
 
+// Near/far decision of one (wave, tile): 0 when every pair of the tile is
+// Newtonian for the wave, else bit c set for each chunk c of the tile whose
+// own box test fails (those chunks go to the near kernel).
+__device__ __forceinline__ unsigned near_mask(const TileMeta *chunks, int t, const TileMeta &tm,
+                                              float d2, double Ox, double Oy, double Oz,
+                                              const float (&bl)[3], const float (&bh)[3],
+                                              float hmax)
+{
+    if (uniform_i(boxes_far(d2, hmax, tm)) != 0) return 0u;
+    unsigned mask = 0;
+    for (int c = 0; c < CHUNKS; ++c) {
+        const TileMeta &cm = chunks[(int64_t)t * CHUNKS + c];
+        if (uniform_i(boxes_far(box_dist2(cm, Ox, Oy, Oz, bl, bh), hmax, cm)) == 0)
+            mask |= 1u << c;
+    }
+    return mask;
+}
+
 template <int WG, int IPL>
 __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const SortedSet &J,
-                                                const TileMeta *tiles, const Block &b,
-                                                unsigned long long *stats, float4 *s_hi,
-                                                float4 *s_lo)
+                                                const TileMeta *tiles, const TileMeta *chunks,
+                                                const Block &b, unsigned long long *stats,
+                                                float4 *s_hi, float4 *s_lo)
 {
     constexpr int WI = 64 * IPL;   // i per wave
     constexpr int GI = WG * IPL;   // i per workgroup
@@ -202,11 +220,12 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
     for (int t = b.t0; t < b.t1; ++t) {
         const TileMeta &tm = tiles[t];
         const float d2 = box_dist2(tm, Ox, Oy, Oz, bl, bh);
-        const bool far = !active || uniform_i(boxes_far(d2, hmax, tm)) != 0;
+        const unsigned mask =
+            active ? near_mask(chunks, t, tm, d2, Ox, Oy, Oz, bl, bh, hmax) : 0u;
         const bool wide = uniform_i(d2 < wide_d2) != 0;
-        if (lane == 0) flags[t - b.t0] = far ? 0 : 1;
-        nearv += far ? 0 : 1;
-        if (WG == 64 && !far) continue;
+        if (lane == 0) flags[t - b.t0] = (unsigned char)mask;
+        nearv += __builtin_popcount(mask);
+        if (WG == 64 && mask == (1u << CHUNKS) - 1) continue;
         for (int s = 0; s < TJ / TS; ++s) {
             __syncthreads();
             for (int jj = tid; jj < TS; jj += WG) {
@@ -218,11 +237,16 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
                                        (float)(rz - (double)hz), 0.0f);
             }
             __syncthreads();
-            if (far && active) {
+            if (!active) continue;
+            for (int h = 0; h < TS / TC; ++h) {
+                const int c = s * (TS / TC) + h;
+                if ((mask >> c) & 1u) continue;
                 if (wide)
-                    far_stage_fp32<IPL, true>(s_hi, s_lo, xh, yh, zh, xl, yl, zl, ax, ay, az, ap);
+                    far_chunk_fp32<IPL, true>(s_hi + h * TC, s_lo + h * TC, xh, yh, zh, xl, yl,
+                                              zl, ax, ay, az, ap);
                 else
-                    far_stage_fp32<IPL, false>(s_hi, s_lo, xh, yh, zh, xl, yl, zl, ax, ay, az, ap);
+                    far_chunk_fp32<IPL, false>(s_hi + h * TC, s_lo + h * TC, xh, yh, zh, xl, yl,
+                                               zl, ax, ay, az, ap);
             }
         }
     }
@@ -236,7 +260,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         b.part[3 * b.pstride + ii[k]] = ap[k];
     }
     if (stats && lane == 0) {
-        atomicAdd(&stats[0], (unsigned long long)(b.t1 - b.t0));
+        atomicAdd(&stats[0], (unsigned long long)(b.t1 - b.t0) * CHUNKS);
         atomicAdd(&stats[1], (unsigned long long)nearv);
     }
 }
@@ -244,8 +268,9 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
 // ------------------------------------------------------- far block, FP64
 template <int WG, int IPL>
 __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedSet &J,
-                                               const TileMeta *tiles, const Block &b,
-                                               unsigned long long *stats, double4 *s_p)
+                                               const TileMeta *tiles, const TileMeta *chunks,
+                                               const Block &b, unsigned long long *stats,
+                                               double4 *s_p)
 {
     constexpr int WI = 64 * IPL;
     constexpr int GI = WG * IPL;
@@ -289,11 +314,13 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
     unsigned char *flags = b.flags + wv * b.fstride;
     for (int t = b.t0; t < b.t1; ++t) {
         const TileMeta &tm = tiles[t];
-        const bool far = !active ||
-                         uniform_i(boxes_far(box_dist2(tm, Ox, Oy, Oz, bl, bh), hmax, tm)) != 0;
-        if (lane == 0) flags[t - b.t0] = far ? 0 : 1;
-        nearv += far ? 0 : 1;
-        if (WG == 64 && !far) continue;
+        const unsigned mask =
+            active ? near_mask(chunks, t, tm, box_dist2(tm, Ox, Oy, Oz, bl, bh), Ox, Oy, Oz, bl,
+                               bh, hmax)
+                   : 0u;
+        if (lane == 0) flags[t - b.t0] = (unsigned char)mask;
+        nearv += __builtin_popcount(mask);
+        if (WG == 64 && mask == (1u << CHUNKS) - 1) continue;
         for (int s = 0; s < TJ / TS; ++s) {
             __syncthreads();
             for (int jj = tid; jj < TS; jj += WG) {
@@ -301,21 +328,24 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
                 s_p[jj] = make_double4(J.x[j] - Ox, J.y[j] - Oy, J.z[j] - Oz, J.m[j]);
             }
             __syncthreads();
-            if (!far || !active) continue;
+            if (!active) continue;
+            for (int h = 0; h < TS / TC; ++h) {
+                if ((mask >> (s * (TS / TC) + h)) & 1u) continue;
 #pragma unroll 2
-            for (int jj = 0; jj < TS; ++jj) {
-                const double4 q = s_p[jj];
+                for (int jj = h * TC; jj < (h + 1) * TC; ++jj) {
+                    const double4 q = s_p[jj];
 #pragma unroll
-                for (int k = 0; k < IPL; ++k) {
-                    const double dx = q.x - xr[k], dy = q.y - yr[k], dz = q.z - zr[k];
-                    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                    const double ri = rsqrt(r2);
-                    const double mr = q.w * ri;
-                    const double mr3 = mr * ri * ri;
-                    ax[k] = fma(mr3, dx, ax[k]);
-                    ay[k] = fma(mr3, dy, ay[k]);
-                    az[k] = fma(mr3, dz, az[k]);
-                    ap[k] -= mr;
+                    for (int k = 0; k < IPL; ++k) {
+                        const double dx = q.x - xr[k], dy = q.y - yr[k], dz = q.z - zr[k];
+                        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                        const double ri = rsqrt(r2);
+                        const double mr = q.w * ri;
+                        const double mr3 = mr * ri * ri;
+                        ax[k] = fma(mr3, dx, ax[k]);
+                        ay[k] = fma(mr3, dy, ay[k]);
+                        az[k] = fma(mr3, dz, az[k]);
+                        ap[k] -= mr;
+                    }
                 }
             }
         }
@@ -330,7 +360,7 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
         b.part[3 * b.pstride + ii[k]] = ap[k];
     }
     if (stats && lane == 0) {
-        atomicAdd(&stats[0], (unsigned long long)(b.t1 - b.t0));
+        atomicAdd(&stats[0], (unsigned long long)(b.t1 - b.t0) * CHUNKS);
         atomicAdd(&stats[1], (unsigned long long)nearv);
     }
 }
@@ -362,15 +392,17 @@ __device__ __forceinline__ void soft_side(double r, double ri, double r2, double
 // with the r == 0 rule documented in include/sphgrav.h.
 // With PSI, ps accumulates m_j (P(q_i) + q_i^2 A(q_i)) over the pairs with
 // r <= H_i (the grad-h softening sum before its -2/H_i^2 factor).
+// The j side's 1/H_j and H_j^2 come precomputed from the LDS stage (one
+// division per staged particle instead of one per pair).
 template <bool FAST, int NK, bool PSI = false>
 __device__ __forceinline__ void near_pair(double dx, double dy, double dz, double Hi,
-                                          double invHi, double Hj, double mj, bool gasj,
-                                          bool same, double &ax, double &ay, double &az,
-                                          double &ap, double *ps = nullptr)
+                                          double invHi, double hi2, double Hj, double invHj,
+                                          double hj2, double mj, bool gasj, bool same,
+                                          double &ax, double &ay, double &az, double &ap,
+                                          double *ps = nullptr)
 {
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const double hj2 = Hj * Hj;
-    if (r2 >= fmax(Hi * Hi, hj2)) {
+    if (r2 >= fmax(hi2, hj2)) {
         const double ri = rsqrt64<FAST>(r2);
         const double mr = mj * ri;
         const double mr3 = mr * ri * ri;
@@ -379,7 +411,7 @@ __device__ __forceinline__ void near_pair(double dx, double dy, double dz, doubl
         az = fma(mr3, dz, az);
         ap -= mr;
     } else if (r2 == 0.0) {
-        if (!same || gasj) ap -= self_coeff(NK) * mj * (invHi + 1.0 / Hj);
+        if (!same || gasj) ap -= self_coeff(NK) * mj * (invHi + invHj);
         if (PSI) *ps -= 2.0 * self_coeff(NK) * mj;   // P(0) = -2c
     } else {
         const double ri = rsqrt64<FAST>(r2);
@@ -387,7 +419,7 @@ __device__ __forceinline__ void near_pair(double dx, double dy, double dz, doubl
         double gacc = 0.0, gpot = 0.0;
         double psk = 0.0;
         soft_side<NK, PSI>(r, ri, r2, Hi, invHi, gacc, gpot, &psk);
-        soft_side<NK>(r, ri, r2, Hj, 1.0 / Hj, gacc, gpot);
+        soft_side<NK>(r, ri, r2, Hj, invHj, gacc, gpot);
         if (PSI) *ps = fma(mj, psk, *ps);
         const double hm = 0.5 * mj;
         const double f = hm * gacc;
@@ -411,13 +443,15 @@ __device__ __forceinline__ bool group_has_near(const Group &g)
     return false;
 }
 
-constexpr int NEAR_CHUNK = 64;   // j per LDS chunk of a near wave (one per lane)
-static_assert(TJ % NEAR_CHUNK == 0, "tile must be a whole number of chunks");
+constexpr int NEAR_CHUNK = TC;   // j per LDS chunk of a near wave (one per lane)
+static_assert(NEAR_CHUNK == 64, "one staged particle per lane");
 
-// 12 KB per workgroup, so LDS does not cap occupancy below the register limit
+// 24 KB per workgroup, so LDS does not cap occupancy below the register limit
 struct NearLds {
     double4 p[NEAR_WAVES][NEAR_CHUNK];   // x, y, z relative to the origin; m
-    double2 h[NEAR_WAVES][NEAR_CHUNK];   // signed H, original index (as double)
+    double4 h[NEAR_WAVES][NEAR_CHUNK];   // signed H, original index (as double), 1/|H|, H^2
+    float4 f[NEAR_WAVES][NEAR_CHUNK];    // FP32 x_hi, y_hi, z_hi, m (FAST kernels)
+    float4 l[NEAR_WAVES][NEAR_CHUNK];    // FP32 x_lo, y_lo, z_lo, H^2 (FAST kernels)
 };
 
 // With PSI (IPL = 1 only) it also forms the grad-h softening term
@@ -433,7 +467,7 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
     const int64_t io = min(g.gbase + 32 * IPL, g.iend - 1);
     const double Ox = I.x[io], Oy = I.y[io], Oz = I.z[io];
 
-    double xr[IPL], yr[IPL], zr[IPL], Hi[IPL], invHi[IPL];
+    double xr[IPL], yr[IPL], zr[IPL], Hi[IPL], invHi[IPL], hi2[IPL];
     double ax[IPL], ay[IPL], az[IPL], ap[IPL], ps[IPL];
     int idi[IPL];
     int64_t ii[IPL];
@@ -448,30 +482,101 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
         zr[k] = I.z[ii[k]] - Oz;
         Hi[k] = fabs(I.Hs[ii[k]]);
         invHi[k] = 1.0 / Hi[k];
+        hi2[k] = Hi[k] * Hi[k];
         idi[k] = I.idx[ii[k]];
         ax[k] = ay[k] = az[k] = ap[k] = ps[k] = 0.0;
     }
 
+    // FP32 hi/lo of the i positions (the FAST kernels' Newtonian pairs)
+    float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL], hi2f[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        xh[k] = (float)xr[k];
+        yh[k] = (float)yr[k];
+        zh[k] = (float)zr[k];
+        xl[k] = (float)(xr[k] - (double)xh[k]);
+        yl[k] = (float)(yr[k] - (double)yh[k]);
+        zl[k] = (float)(zr[k] - (double)zh[k]);
+        hi2f[k] = (float)hi2[k];
+    }
+
     int cnt = 0;
     for (int t = g.t0; t < g.t1; ++t) {
-        if (!g.flags[t - g.t0]) continue;
+        const unsigned fl = g.flags[t - g.t0];
+        if (!fl) continue;
         if ((cnt++ % NEAR_WAVES) != w) continue;
         for (int c0 = 0; c0 < TJ; c0 += NEAR_CHUNK) {
+            if (!((fl >> (c0 / TC)) & 1u)) continue;   // a far chunk of a near tile
             const int64_t j = (int64_t)t * TJ + c0 + lane;
             wave_lds_sync();
-            lds.p[w][lane] = make_double4(J.x[j] - Ox, J.y[j] - Oy, J.z[j] - Oz, J.m[j]);
-            lds.h[w][lane] = make_double2(J.Hs[j], (double)J.idx[j]);
+            const double jx = J.x[j] - Ox, jy = J.y[j] - Oy, jz = J.z[j] - Oz;
+            lds.p[w][lane] = make_double4(jx, jy, jz, J.m[j]);
+            const double hs = J.Hs[j], ha = fabs(hs);
+            lds.h[w][lane] = make_double4(hs, (double)J.idx[j], 1.0 / ha, ha * ha);
+            if (FAST) {
+                const float hx = (float)jx, hy = (float)jy, hz = (float)jz;
+                lds.f[w][lane] = make_float4(hx, hy, hz, (float)J.m[j]);
+                lds.l[w][lane] = make_float4((float)(jx - (double)hx), (float)(jy - (double)hy),
+                                             (float)(jz - (double)hz), (float)(ha * ha));
+            }
             wave_lds_sync();
-#pragma unroll 1
-            for (int jj = 0; jj < NEAR_CHUNK; ++jj) {
-                const double4 p = lds.p[w][jj];
-                const double2 h = lds.h[w][jj];
+            if (FAST) {
+                // Newtonian pairs in FP32 (hi+lo differences, sums folded into
+                // FP64 every FOLD j); the pairs with r^2 < max(H_i^2, H_j^2),
+                // including r = 0, take the FP64 pair on their lanes only.
+                for (int f0 = 0; f0 < NEAR_CHUNK; f0 += FOLD) {
+                    float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
 #pragma unroll
-                for (int k = 0; k < IPL; ++k)
-                    near_pair<FAST, NK, PSI>(p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k],
-                                             invHi[k], fabs(h.x), p.w, h.x > 0.0,
-                                             (int)h.y == idi[k], ax[k], ay[k], az[k], ap[k],
-                                             &ps[k]);
+                    for (int k = 0; k < IPL; ++k) fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
+#pragma unroll 2
+                    for (int jj = f0; jj < f0 + FOLD; ++jj) {
+                        const float4 a = lds.f[w][jj];
+                        const float4 b = lds.l[w][jj];
+#pragma unroll
+                        for (int k = 0; k < IPL; ++k) {
+                            const float dx = (a.x - xh[k]) + (b.x - xl[k]);
+                            const float dy = (a.y - yh[k]) + (b.y - yl[k]);
+                            const float dz = (a.z - zh[k]) + (b.z - zl[k]);
+                            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                            const bool soft = r2 < fmaxf(hi2f[k], b.w);
+                            const float ri = soft ? 0.0f : __builtin_amdgcn_rsqf(r2);
+                            const float mr = a.w * ri;
+                            const float mr3 = mr * ri * ri;
+                            fx[k] = fmaf(mr3, dx, fx[k]);
+                            fy[k] = fmaf(mr3, dy, fy[k]);
+                            fz[k] = fmaf(mr3, dz, fz[k]);
+                            fp[k] += mr;
+                            if (soft) {
+                                const double4 p = lds.p[w][jj];
+                                const double4 h = lds.h[w][jj];
+                                near_pair<FAST, NK, PSI>(p.x - xr[k], p.y - yr[k], p.z - zr[k],
+                                                         Hi[k], invHi[k], hi2[k], fabs(h.x), h.z,
+                                                         h.w, p.w, h.x > 0.0,
+                                                         (int)h.y == idi[k], ax[k], ay[k], az[k],
+                                                         ap[k], &ps[k]);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < IPL; ++k) {
+                        ax[k] += (double)fx[k];
+                        ay[k] += (double)fy[k];
+                        az[k] += (double)fz[k];
+                        ap[k] -= (double)fp[k];
+                    }
+                }
+            } else {
+#pragma unroll 1
+                for (int jj = 0; jj < NEAR_CHUNK; ++jj) {
+                    const double4 p = lds.p[w][jj];
+                    const double4 h = lds.h[w][jj];
+#pragma unroll
+                    for (int k = 0; k < IPL; ++k)
+                        near_pair<FAST, NK, PSI>(p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k],
+                                                 invHi[k], hi2[k], fabs(h.x), h.z, h.w, p.w,
+                                                 h.x > 0.0, (int)h.y == idi[k], ax[k], ay[k],
+                                                 az[k], ap[k], &ps[k]);
+                }
             }
         }
     }
@@ -539,14 +644,15 @@ __global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
 {
     __shared__ float4 s_hi[TS];
     __shared__ float4 s_lo[TS];
-    far_block_fp32c<WG, IPL>(a.i, a.j, a.tiles, single_block<WG, IPL>(a), a.stats, s_hi, s_lo);
+    far_block_fp32c<WG, IPL>(a.i, a.j, a.tiles, a.chunks, single_block<WG, IPL>(a), a.stats, s_hi,
+                             s_lo);
 }
 
 template <int WG, int IPL>
 __global__ __launch_bounds__(WG) void k_far_fp64(PairArgs a)
 {
     __shared__ double4 s_p[TS];
-    far_block_fp64<WG, IPL>(a.i, a.j, a.tiles, single_block<WG, IPL>(a), a.stats, s_p);
+    far_block_fp64<WG, IPL>(a.i, a.j, a.tiles, a.chunks, single_block<WG, IPL>(a), a.stats, s_p);
 }
 
 template <int IPL, bool FAST, int NK>
@@ -585,6 +691,64 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_psi(PairArgs a)
     g.psipart = a.psipart + (int64_t)jp * a.ni;
     if (g.gbase >= g.iend || !group_has_near(g)) return;
     near_group<1, FAST, NK, true>(a.i, a.j, g, lds, s_red);
+}
+
+// ------------------------------------------------- grad-h term on its own
+// psi_i = -(2/H_i^2) sum_{j: r <= H_i} m_j (P(q_i) + q_i^2 A(q_i)) of the
+// gas particles of the i-set, without the accelerations: one wave per
+// 64-particle i-group tests every tile and chunk against the group's box
+// (r <= H_i needs only the wave's largest H) and sums the chunks it cannot
+// rule out, staged through its own LDS slice.  FP64 throughout.
+template <int NK>
+__global__ __launch_bounds__(256) void k_psi_only(PairArgs a)
+{
+    __shared__ double4 s_p[4][TC];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t gi = (int64_t)blockIdx.x * 4 + w;
+    if (gi >= a.ngroups) return;   // whole wave; no block barrier below
+    const int64_t gbase = gi * 64;
+    const int64_t io = min(gbase + 32, a.ni - 1);
+    const double Ox = a.i.x[io], Oy = a.i.y[io], Oz = a.i.z[io];
+    const bool valid = gbase + lane < a.ni;
+    const int64_t i = valid ? gbase + lane : a.ni - 1;
+    const double xr = a.i.x[i] - Ox, yr = a.i.y[i] - Oy, zr = a.i.z[i] - Oz;
+    const double Hs = a.i.Hs[i], Hi = fabs(Hs), invHi = 1.0 / Hi, hi2 = Hi * Hi;
+    float bl[3] = {(float)xr, (float)yr, (float)zr}, bh[3] = {bl[0], bl[1], bl[2]};
+    for (int c = 0; c < 3; ++c) {
+        bl[c] = wave_min(bl[c]);
+        bh[c] = wave_max(bh[c]);
+    }
+    // only side i matters for psi: a chunk farther than the wave's largest
+    // H_i (with the far test's margin) holds no contributing pair
+    const float hmax = wave_max((float)Hi);
+    const float lim2 = hmax * hmax * (1.0f + FAR_MARGIN);
+    double ps = 0.0;
+    for (int t = 0; t < a.ntiles; ++t) {
+        if (uniform_i(box_dist2(a.tiles[t], Ox, Oy, Oz, bl, bh) <= lim2) == 0) continue;
+        for (int c = 0; c < CHUNKS; ++c) {
+            const TileMeta &cm = a.chunks[(int64_t)t * CHUNKS + c];
+            if (uniform_i(box_dist2(cm, Ox, Oy, Oz, bl, bh) <= lim2) == 0) continue;
+            const int64_t j = (int64_t)t * TJ + c * TC + lane;
+            wave_lds_sync();
+            s_p[w][lane] = make_double4(a.j.x[j] - Ox, a.j.y[j] - Oy, a.j.z[j] - Oz, a.j.m[j]);
+            wave_lds_sync();
+#pragma unroll 2
+            for (int jj = 0; jj < TC; ++jj) {
+                const double4 p = s_p[w][jj];
+                const double dx = p.x - xr, dy = p.y - yr, dz = p.z - zr;
+                const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                if (r2 <= hi2) {
+                    const double r = sqrt(r2), q = r * invHi;
+                    double A, P;
+                    poly_AP(NK, q, q > 0.0 ? 1.0 / q : 0.0, A, P);
+                    ps = fma(p.w, fma(q * q, A, P), ps);
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    const int64_t o = (int64_t)a.i.idx[i] - a.ibeg;
+    a.psi[o] = Hs > 0.0 ? -2.0 * invHi * invHi * ps : 0.0;
 }
 
 // Fixed-order FP64 sum of the j-partition partials, scattered to input order.
@@ -646,9 +810,9 @@ __global__ __launch_bounds__(WG) void k_far_batch(BatchArgs a)
         if (item >= a.nitems) break;
         const Block b = batch_block<WG, IPL>(a, item);
         if (FP64)
-            far_block_fp64<WG, IPL>(a.j, a.j, a.tiles, b, a.stats, s_buf);
+            far_block_fp64<WG, IPL>(a.j, a.j, a.tiles, a.chunks, b, a.stats, s_buf);
         else
-            far_block_fp32c<WG, IPL>(a.j, a.j, a.tiles, b, a.stats, s_hi, s_lo);
+            far_block_fp32c<WG, IPL>(a.j, a.j, a.tiles, a.chunks, b, a.stats, s_hi, s_lo);
     }
 }
 
@@ -773,6 +937,19 @@ bool pair_config_supported(int precision, int wg, int ipl)
     const bool wg_ok = wg == 64 || wg == 128 || wg == 256;
     if (precision == 0) return wg_ok && (ipl == 1 || ipl == 2 || ipl == 4);
     return wg_ok && (ipl == 1 || ipl == 2);
+}
+
+hipError_t launch_psi_only(const PairArgs &a, hipStream_t s)
+{
+    if (a.ni <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((a.ngroups + 3) / 4)), block(256);
+    switch (a.nkernel) {
+    case 0: hipLaunchKernelGGL((k_psi_only<0>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_psi_only<1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_psi_only<2>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_partials(const PairArgs &a, hipStream_t s)

@@ -13,7 +13,10 @@ namespace sg {
 constexpr int TJ = 256;
 
 // Per j-tile bounding box (FP64, absolute coordinates) and largest support
-// radius.  64 bytes.
+// radius.  64 bytes.  The same record describes each of the TJ / TC chunks
+// of TC consecutive particles of a tile, the unit of the near/far split.
+constexpr int TC = 64;
+constexpr int CHUNKS = 4;   // TJ / TC
 struct TileMeta {
     double lo[3];
     double hi[3];
@@ -38,6 +41,7 @@ struct PairArgs {
     SortedSet j;             // all n particles
     SortedSet i;             // i-range particles in sorted order (may alias j)
     const TileMeta *tiles;   // padded/TJ entries
+    const TileMeta *chunks;  // padded/TC entries
     int64_t ni;              // i-particles
     int ibeg;                // outputs go to out[idx - ibeg]
     int nkernel;
@@ -48,8 +52,9 @@ struct PairArgs {
     int64_t ngroups;         // i-groups (waves of the far kernel)
     double *ax, *ay, *az, *pot;   // final outputs, input order, length ni
     double *partial;              // [jsplit][4][ni] sorted-order partial sums
-    unsigned char *nearflag;      // [ngroups][ntiles]: 1 = tile takes the softened path
-    unsigned long long *stats;    // [2]: tile visits, near visits (may be null)
+    unsigned char *nearflag;      // [ngroups][ntiles]: bit c set = chunk c of the tile may
+                                  // hold softened pairs and takes the near kernel
+    unsigned long long *stats;    // [2]: chunk visits, near chunk visits (may be null)
     double *psi;                  // grad-h softening term, input order (null: not formed)
     double *psipart;              // [jsplit][ni] its partial sums (with psi)
 };
@@ -68,6 +73,7 @@ struct BatchSet {
 struct BatchArgs {
     SortedSet j;                 // padded sorted arrays of all sets (also the i-set)
     const TileMeta *tiles;
+    const TileMeta *chunks;
     const BatchSet *sets;
     int nsets;
     const int2 *items;           // (set, i-block), longest first
@@ -112,6 +118,8 @@ bool adj_config_supported(int wg);
 hipError_t launch_pair_far(int precision, const PairArgs &a, int wg, hipStream_t s);
 hipError_t launch_pair_near(int precision, const PairArgs &a, hipStream_t s);
 hipError_t launch_reduce_partials(const PairArgs &a, hipStream_t s);
+// Grad-h term alone (a.psi, no accelerations): groups of 64 i, a.ngroups of them.
+hipError_t launch_psi_only(const PairArgs &a, hipStream_t s);
 bool pair_config_supported(int precision, int wg, int ipl);
 // Batch: queue reset, persistent far kernel on nwg workgroups, near kernel,
 // scatter.  Contains no memset, so it can be captured in a graph.
@@ -132,6 +140,7 @@ struct PrepBuffers {
     double *sx, *sy, *sz, *sm, *sH;
     int *sidx;
     TileMeta *tiles;
+    TileMeta *chunks;     // CHUNKS per tile
     // i-subset sorted arrays
     double *ix, *iy, *iz, *im, *iH;
     int *iidx;
@@ -155,7 +164,8 @@ hipError_t launch_gather_cot(const double *wx, const double *wy, const double *w
                              double *owx, double *owy, double *owz, double *ou,
                              const PrepBuffers &b, hipStream_t s);
 hipError_t launch_tile_meta(const double *sx, const double *sy, const double *sz,
-                            const double *sH, int ntiles, TileMeta *tiles, hipStream_t s);
+                            const double *sH, int ntiles, TileMeta *tiles, TileMeta *chunks,
+                            hipStream_t s);
 hipError_t launch_fill_zero_int(int *p, int n, hipStream_t s);
 // Batch preparation: keys = (set << 42) | top 42 bits of the Morton code, and
 // the gather into the padded per-set layout.

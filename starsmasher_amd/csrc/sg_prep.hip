@@ -9,7 +9,8 @@
 //   4. gather into sorted FP64 structure-of-arrays, H = sqrt|range2| signed
 //      by the gas/point flag (:603-605), padded to a multiple of TJ with
 //      massless copies of the last particle;
-//   5. per-tile bounding box and largest H (near/far test of the pair kernel).
+//   5. per-tile and per-chunk (64 particles) bounding box and largest H (the
+//      near/far test of the pair kernels).
 #include "sg_internal.h"
 
 #include <math.h>
@@ -186,23 +187,39 @@ __global__ __launch_bounds__(TJ) void k_tile_meta(const double *__restrict__ sx,
                                                   const double *__restrict__ sy,
                                                   const double *__restrict__ sz,
                                                   const double *__restrict__ sH,
-                                                  TileMeta *__restrict__ tiles)
+                                                  TileMeta *__restrict__ tiles,
+                                                  TileMeta *__restrict__ chunks)
 {
     static_assert(TJ == RB, "one thread per tile particle");
+    static_assert(TC == 64 && CHUNKS * TC == TJ, "one wave per chunk");
     const int64_t k = (int64_t)blockIdx.x * TJ + threadIdx.x;
     double v[6] = {sx[k], sy[k], sz[k], sx[k], sy[k], sz[k]};
     double h = fabs(sH[k]);
+    // chunk box: one wave per TC particles
+    double c[6];
+    for (int q = 0; q < 3; ++q) c[q] = warp_min(v[q]);
+    for (int q = 3; q < 6; ++q) c[q] = warp_max(v[q]);
+    h = warp_max(h);
+    if ((threadIdx.x & 63) == 0) {
+        TileMeta t;
+        for (int q = 0; q < 3; ++q) {
+            t.lo[q] = c[q];
+            t.hi[q] = c[3 + q];
+        }
+        t.maxH = h;
+        t.pad = 0.0;
+        chunks[(int64_t)blockIdx.x * CHUNKS + (threadIdx.x >> 6)] = t;
+    }
     block_minmax6(v);
     __shared__ double shH[TJ / 64];
-    h = warp_max(h);
     if ((threadIdx.x & 63) == 0) shH[threadIdx.x >> 6] = h;
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < TJ / 64; ++w) h = fmax(h, shH[w]);
         TileMeta t;
-        for (int c = 0; c < 3; ++c) {
-            t.lo[c] = v[c];
-            t.hi[c] = v[3 + c];
+        for (int q = 0; q < 3; ++q) {
+            t.lo[q] = v[q];
+            t.hi[q] = v[3 + q];
         }
         t.maxH = h;
         t.pad = 0.0;
@@ -353,10 +370,11 @@ hipError_t launch_gather_cot(const double *wx, const double *wy, const double *w
 }
 
 hipError_t launch_tile_meta(const double *sx, const double *sy, const double *sz,
-                            const double *sH, int ntiles, TileMeta *tiles, hipStream_t s)
+                            const double *sH, int ntiles, TileMeta *tiles, TileMeta *chunks,
+                            hipStream_t s)
 {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_meta, dim3(ntiles), dim3(TJ), 0, s, sx, sy, sz, sH, tiles);
+    hipLaunchKernelGGL(k_tile_meta, dim3(ntiles), dim3(TJ), 0, s, sx, sy, sz, sH, tiles, chunks);
     return hipGetLastError();
 }
 

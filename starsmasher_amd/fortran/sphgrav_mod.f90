@@ -19,6 +19,7 @@ module sphgrav
 
   public :: firsthalf_grav_forces, lasthalf_grav_forces, gpu_init_dev
   public :: sg_init_device, sg_firsthalf, sg_lasthalf, sg_last_status, sg_device_count
+  public :: sg_grav_psi
   public :: sg_status_message
 
   interface
@@ -60,6 +61,16 @@ module sphgrav
        import :: c_int, c_double
        real(c_double), intent(inout) :: ax(*), ay(*), az(*), pot(*)
      end function sg_lasthalf
+
+     ! grad-h softening term psi of particles n_lower..n_lower+my_length-1
+     ! (advance.f90:716-737), written to psi(n_lower:...); blocking
+     integer(c_int) function sg_grav_psi(n, n_lower, my_length, x, y, z, m, range2, nkernel, psi) &
+          bind(C, name='sg_grav_psi')
+       import :: c_int, c_double
+       integer(c_int), value :: n, n_lower, my_length, nkernel
+       real(c_double), intent(in) :: x(*), y(*), z(*), m(*), range2(*)
+       real(c_double), intent(inout) :: psi(*)
+     end function sg_grav_psi
 
      integer(c_int) function sg_last_status() bind(C, name='sg_last_status')
        import :: c_int

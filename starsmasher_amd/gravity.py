@@ -97,6 +97,25 @@ def gravforce(x, y, z, am, hp, u, ngrav_lower=1, ngrav_upper=None, q=1, nkernel=
                           nkernel)
 
 
+def bonet_psi(x, y, z, am, hp, u, nkernel=0, ngrav_lower=1, ngrav_upper=None):
+    """The grad-h softening term psi_i of particles ngrav_lower..ngrav_upper
+    (1-based; advance.f90:716-737's bonet1_psi, with nselfgravity = 1), from
+    the GPU through sg_grav_psi.  Returns an array of length ntot with the
+    i-range filled and zeros elsewhere."""
+    ntot = len(x)
+    ngrav_upper = ntot if ngrav_upper is None else int(ngrav_upper)
+    hp = np.asarray(hp, dtype=np.float64)
+    rng = 4.0 * hp * hp
+    rng = np.where(np.asarray(u) == 0.0, -rng, rng)
+    ins = [_host_f64(a, nm, ntot) for a, nm in ((x, "x"), (y, "y"), (z, "z"), (am, "am"),
+                                                 (rng, "range"))]
+    out = np.zeros(ntot)
+    check(_lib.load().sg_grav_psi(ntot, int(ngrav_lower), ngrav_upper - ngrav_lower + 1,
+                                  *[_ptr(a) for a in ins], int(nkernel), _ptr(out)),
+          "sg_grav_psi")
+    return out
+
+
 def _dev_ptr(t, name, n=None):
     """Device pointer of a :class:`starsmasher_amd.hipmem.DeviceArray`, a raw
     integer address, or a torch tensor (float64, contiguous, on a GPU; only
@@ -197,6 +216,18 @@ class Gravity:
                                          int(nkernel), int(precision), *[_ptr(o) for o in outs])
         check(st, "sg_compute_psi_host")
         return tuple(outs)
+
+    def psi(self, x, y, z, m, range2, nkernel=0, ibeg=0, ni=None):
+        """Blocking grad-h term alone (FP64) of the 0-based i-range."""
+        n = len(x)
+        ni = n - ibeg if ni is None else int(ni)
+        ins = [_host_f64(a, nm, n) for a, nm in ((x, "x"), (y, "y"), (z, "z"), (m, "m"),
+                                                  (range2, "range2"))]
+        out = np.empty(max(ni, 0))
+        st = self._L.sg_psi_host(self._h, n, int(ibeg), ni, *[_ptr(a) for a in ins],
+                                 int(nkernel), _ptr(out))
+        check(st, "sg_psi_host")
+        return out
 
     def vjp(self, x, y, z, m, range2, wx, wy, wz, u, nkernel=0, precision=SG_PREC_FP32C):
         """Blocking host-array adjoint: the gradient of

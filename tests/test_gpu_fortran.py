@@ -27,10 +27,10 @@ def _write_input(path, d, nlo, nhi, nkernel):
             np.ascontiguousarray(d[k], dtype="<f8").tofile(f)
 
 
-def _read_output(path, n):
+def _read_output(path, n, narrays=4):
     a = np.fromfile(path, dtype="<f8")
-    assert a.size == 4 * n
-    return a.reshape(4, n)
+    assert a.size == narrays * n
+    return a.reshape(narrays, n)
 
 
 @pytest.mark.parametrize("exe", ["gravity_driver", "sphgrav_check"])
@@ -51,7 +51,7 @@ def test_fortran_callers(tmp_path, exe, nlo, nhi, nk):
     assert r.returncode == 0, r.stderr
     if exe == "sphgrav_check":
         assert "status-check ok" in r.stdout
-    out = _read_output(fout, 3000)
+    out = _read_output(fout, 3000, 5 if exe == "sphgrav_check" else 4)
     lo, hi = nlo - 1, nhi
     assert np.all(out[:, :lo] == -7777.0) and np.all(out[:, hi:] == -7777.0)
     ref = oracle.direct(d["x"], d["y"], d["z"], d["m"], d["range2"], nkernel=nk,
@@ -59,3 +59,11 @@ def test_fortran_callers(tmp_path, exe, nlo, nhi, nk):
     ea, ep = rel_errors(out[:3, lo:hi].T, out[3, lo:hi], np.stack(ref[:3], -1), ref[3])
     print(f"[fortran {exe}] acc max {ea.max():.3e} pot max {ep.max():.3e}")
     assert ea.max() < 1e-5 and ep.max() < 1e-6
+    if exe == "sphgrav_check":
+        psi = out[4]
+        assert np.all(psi[:lo] == -7777.0) and np.all(psi[hi:] == -7777.0)
+        want = oracle.psi(d["x"], d["y"], d["z"], d["m"], d["range2"], nkernel=nk,
+                          rows=np.arange(lo, hi))
+        gas = want != 0.0
+        assert np.all(psi[lo:hi][~gas] == 0.0)
+        assert np.max(np.abs(psi[lo:hi][gas] - want[gas]) / np.abs(want[gas])) < 1e-11
