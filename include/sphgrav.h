@@ -132,7 +132,7 @@ int sg_compute_host(sg_context *ctx, int n, int ibeg, int ni,
 
 /* Tile configuration of the pair kernel for a precision.
  * wg: threads per workgroup (64, 128 or 256); ipl: i-particles per lane
- * (1, 2 or 4); jsplit: number of j-partitions per i-block (>= 1, partial sums
+ * (1 or 2); jsplit: number of j-partitions per i-block (>= 1, partial sums
  * are reduced in FP64 in a fixed order); 0 for jsplit selects it from n. */
 int sg_set_config(sg_context *ctx, int precision, int wg, int ipl, int jsplit);
 int sg_get_config(sg_context *ctx, int precision, int *wg, int *ipl, int *jsplit);

@@ -16,7 +16,7 @@ from dataclasses import dataclass, field
 from ._lib import SG_PREC_FP32C, SG_PREC_FP64
 
 CANDIDATES = {
-    SG_PREC_FP32C: [(64, 1), (64, 2), (64, 4), (128, 1), (128, 2), (256, 1), (256, 2), (256, 4)],
+    SG_PREC_FP32C: [(64, 1), (64, 2), (128, 1), (128, 2), (256, 1), (256, 2)],
     SG_PREC_FP64: [(64, 1), (64, 2), (128, 1), (128, 2), (256, 1), (256, 2)],
 }
 

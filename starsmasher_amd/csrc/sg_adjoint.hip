@@ -42,7 +42,7 @@ constexpr int AFOLD = 32;       // FP32 partial sums span AFOLD j
 constexpr int ANEAR_WAVES = 4;  // waves sharing one group's near tiles
 constexpr int ACHUNK = 64;      // j per LDS chunk of a near wave
 
-static_assert(TJ % ATS == 0 && ATS % AFOLD == 0 && TJ % ACHUNK == 0, "tile shapes");
+static_assert(TJ % ATS == 0 && ATS % TC == 0 && TC % AFOLD == 0 && ACHUNK == TC, "tile shapes");
 
 // dA/dq divided by q: a polynomial for the Wendland kernels, the cubic
 // spline's outer branch carries q^-4.  Term-by-term derivatives of the
@@ -171,17 +171,92 @@ __device__ __forceinline__ void adj_pair64(double dx, double dy, double dz, doub
 // ------------------------------------------------------------ far kernel
 // i-side state of one lane in the FP32 far kernel.
 struct Lane32 {
-    float xh, yh, zh, xl, yl, zl, m, wx, wy, wz, u;
+    float xh, yh, zh, xl, yl, zl, m, wx, wy, wz, u, h2;   // h2 = H_k^2
 };
 
-// One LDS stage of far pairs in FP32.  acc: G1 (3), G2, G3 (3), GM' with
-// GM' = sum (u_l / r + w_l.dr / r^3) = -GM.
-template <bool WIDE>
-__device__ __forceinline__ void adj_stage_fp32(const float4 *s_p, const float4 *s_w,
-                                               const float4 *s_lo, const Lane32 &q,
-                                               double (&acc)[8])
+// One near chunk (TC j) of the FP32C adjoint inside the far kernel: the FP32
+// pair body with the lanes whose pair may be softened (r^2 < max(H_k^2,
+// H_l^2) in FP32, r = 0 included) masked out, then adj_pair64 in FP64 for
+// those lanes of the j a wave ballot marked.
+template <int NK>
+__device__ __forceinline__ void adj_near_chunk_fp32(const float4 *s_p, const float4 *s_w,
+                                                    const float4 *s_lo, const Lane32 &q,
+                                                    const AdjArgs &a, int64_t jbase, int64_t i,
+                                                    double Ox, double Oy, double Oz,
+                                                    double (&acc)[9])
 {
-    for (int j0 = 0; j0 < ATS; j0 += AFOLD) {
+    uint64_t jmask = 0;
+    for (int j0 = 0; j0 < TC; j0 += AFOLD) {
+        float g1x = 0.0f, g1y = 0.0f, g1z = 0.0f, g2 = 0.0f;
+        float g3x = 0.0f, g3y = 0.0f, g3z = 0.0f, gm = 0.0f;
+        for (int jj = j0; jj < j0 + AFOLD; ++jj) {
+            const float4 p = s_p[jj];
+            const float4 w = s_w[jj];
+            const float4 l = s_lo[jj];   // l.w = H_l^2
+            const float dx = (p.x - q.xh) + (l.x - q.xl);
+            const float dy = (p.y - q.yh) + (l.y - q.yl);
+            const float dz = (p.z - q.zh) + (l.z - q.zl);
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const bool soft = r2 < fmaxf(q.h2, l.w);
+            const float ri = soft ? 0.0f : __builtin_amdgcn_rsqf(r2);
+            const float ri2 = ri * ri;
+            const float ri3 = ri2 * ri;
+            const float wkdr = fmaf(q.wx, dx, fmaf(q.wy, dy, q.wz * dz));
+            const float wldr = fmaf(w.x, dx, fmaf(w.y, dy, w.z * dz));
+            const float t = fmaf(p.w, wkdr, -q.m * wldr);
+            const float sc = fmaf(p.w, q.u, q.m * w.w);
+            const float c = ri3 * fmaf(3.0f * t, ri2, -sc);
+            g1x = fmaf(ri3, w.x, g1x);
+            g1y = fmaf(ri3, w.y, g1y);
+            g1z = fmaf(ri3, w.z, g1z);
+            g2 = fmaf(ri3, p.w, g2);
+            g3x = fmaf(c, dx, g3x);
+            g3y = fmaf(c, dy, g3y);
+            g3z = fmaf(c, dz, g3z);
+            gm = fmaf(w.w, ri, fmaf(ri3, wldr, gm));
+            if (__ballot(soft)) jmask |= 1ull << jj;
+        }
+        acc[0] += (double)g1x;
+        acc[1] += (double)g1y;
+        acc[2] += (double)g1z;
+        acc[3] += (double)g2;
+        acc[4] += (double)g3x;
+        acc[5] += (double)g3y;
+        acc[6] += (double)g3z;
+        acc[7] += (double)gm;
+    }
+    const SortedSet &P = a.p;
+    while (jmask) {
+        const int jj = __builtin_ctzll(jmask);
+        jmask &= jmask - 1;
+        const float4 p = s_p[jj];
+        const float4 l = s_lo[jj];
+        const float dx = (p.x - q.xh) + (l.x - q.xl);
+        const float dy = (p.y - q.yh) + (l.y - q.yl);
+        const float dz = (p.z - q.zh) + (l.z - q.zl);
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        if (!(r2 < fmaxf(q.h2, l.w))) continue;   // this lane took the FP32 pair
+        const int64_t j = jbase + jj;
+        const double Hs = P.Hs[i], Hk = fabs(Hs), hl = fabs(P.Hs[j]);
+        Acc9 g = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        adj_pair64<true, NK>((P.x[j] - Ox) - (P.x[i] - Ox), (P.y[j] - Oy) - (P.y[i] - Oy),
+                             (P.z[j] - Oz) - (P.z[i] - Oz), Hk, 1.0 / Hk, P.m[i], a.wx[i],
+                             a.wy[i], a.wz[i], a.u[i], Hs > 0.0, hl, P.m[j], a.wx[j], a.wy[j],
+                             a.wz[j], a.u[j], P.idx[j] == P.idx[i], g);
+        for (int k = 0; k < 7; ++k) acc[k] += g[k];
+        acc[7] -= g[7];
+        acc[8] += g[8];
+    }
+}
+
+// One chunk (TC j) of far pairs in FP32.  acc: G1 (3), G2, G3 (3), GM' with
+// GM' = sum (u_l / r + w_l.dr / r^3) = -GM, and GH.
+template <bool WIDE>
+__device__ __forceinline__ void adj_chunk_fp32(const float4 *s_p, const float4 *s_w,
+                                               const float4 *s_lo, const Lane32 &q,
+                                               double (&acc)[9])
+{
+    for (int j0 = 0; j0 < TC; j0 += AFOLD) {
         float g1x = 0.0f, g1y = 0.0f, g1z = 0.0f, g2 = 0.0f;
         float g3x = 0.0f, g3y = 0.0f, g3z = 0.0f, gm = 0.0f;
 #pragma unroll 2
@@ -225,13 +300,13 @@ __device__ __forceinline__ void adj_stage_fp32(const float4 *s_p, const float4 *
 }
 
 // The same in FP64 (the plain FP64 comparison path).
-__device__ __forceinline__ void adj_stage_fp64(const double4 *s_p, const double4 *s_w,
+__device__ __forceinline__ void adj_chunk_fp64(const double4 *s_p, const double4 *s_w,
                                                double xr, double yr, double zr, double mk,
                                                double wkx, double wky, double wkz, double uk,
-                                               double (&acc)[8])
+                                               double (&acc)[9])
 {
 #pragma unroll 2
-    for (int jj = 0; jj < ATS; ++jj) {
+    for (int jj = 0; jj < TC; ++jj) {
         const double4 p = s_p[jj];
         const double4 w = s_w[jj];
         const double dx = p.x - xr, dy = p.y - yr, dz = p.z - zr;
@@ -261,12 +336,15 @@ __device__ __forceinline__ void adj_tile_range(const AdjArgs &a, int jp, int &t0
     t1 = (int)((int64_t)a.ntiles * (jp + 1) / a.jsplit);
 }
 
-// Grid: niblocks x jsplit workgroups of WG lanes, one i per lane.
-template <int WG, bool FP64>
-__global__ __launch_bounds__(WG) void k_adj_far(AdjArgs a)
+// Grid: niblocks x jsplit workgroups of WG lanes, one i per lane.  FP32C
+// (FP64 = false) takes the near chunks itself; FP64 flags them (chunk
+// bits) for k_adj_near.
+template <int WG, bool FP64, int NK>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(FP64 ? 1 : 4)))
+void k_adj_far(AdjArgs a)
 {
     // FP64: {x, y, z, m} and {wx, wy, wz, u} as double4; FP32: {x_hi, y_hi,
-    // z_hi, m}, {wx, wy, wz, u} and {x_lo, y_lo, z_lo, 0} as float4 (6 KB)
+    // z_hi, m}, {wx, wy, wz, u} and {x_lo, y_lo, z_lo, H^2} as float4 (6 KB)
     __shared__ double4 s_buf[2 * ATS];
     double4 *s_p64 = s_buf, *s_w64 = s_buf + ATS;
     float4 *s_p32 = reinterpret_cast<float4 *>(s_buf);
@@ -288,6 +366,7 @@ __global__ __launch_bounds__(WG) void k_adj_far(AdjArgs a)
     const int64_t i = valid ? i0 : iend - 1;
     const double rx = P.x[i] - Ox, ry = P.y[i] - Oy, rz = P.z[i] - Oz;
     const double mk = P.m[i], wkx = a.wx[i], wky = a.wy[i], wkz = a.wz[i], uk = a.u[i];
+    const double Hd = fabs(P.Hs[i]);
 
     Lane32 q;
     q.xh = (float)rx;
@@ -301,6 +380,7 @@ __global__ __launch_bounds__(WG) void k_adj_far(AdjArgs a)
     q.wy = (float)wky;
     q.wz = (float)wkz;
     q.u = (float)uk;
+    q.h2 = (float)(Hd * Hd);
 
     // the wave's box and largest support radius (the forward kernels' test)
     float bl[3] = {q.xh, q.yh, q.zh}, bh[3] = {q.xh, q.yh, q.zh};
@@ -310,21 +390,22 @@ __global__ __launch_bounds__(WG) void k_adj_far(AdjArgs a)
         bh[c] = wave_max(bh[c]);
         reach = fmaxf(reach, fmaxf(-bl[c], bh[c]));
     }
-    const float hmax = wave_max((float)fabs(P.Hs[i]));
+    const float hmax = wave_max((float)Hd);
     const float wide_d2 = 3.0f * (WIDE_K * reach) * (WIDE_K * reach);
     const bool active = uniform_i(ibase + wv * 64 < iend) != 0;
     unsigned char *flags = a.nearflag + ((int64_t)ib * (WG / 64) + wv) * a.ntiles;
 
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     unsigned nearv = 0;
     for (int t = t0; t < t1; ++t) {
         const TileMeta &tm = a.tiles[t];
         const float d2 = box_dist2(tm, Ox, Oy, Oz, bl, bh);
-        const bool far = !active || uniform_i(boxes_far(d2, hmax, tm)) != 0;
+        const unsigned mask =
+            active ? near_mask(a.chunks, t, tm, d2, Ox, Oy, Oz, bl, bh, hmax) : 0u;
         const bool wide = uniform_i(d2 < wide_d2) != 0;
-        if (lane == 0) flags[t] = far ? 0 : 1;
-        nearv += far ? 0 : 1;
-        if (WG == 64 && !far) continue;
+        if (FP64 && lane == 0) flags[t] = (unsigned char)mask;
+        nearv += __builtin_popcount(mask);
+        if (FP64 && WG == 64 && mask == (1u << CHUNKS) - 1) continue;
         for (int st = 0; st < TJ / ATS; ++st) {
             __syncthreads();
             for (int jj = tid; jj < ATS; jj += WG) {
@@ -335,21 +416,32 @@ __global__ __launch_bounds__(WG) void k_adj_far(AdjArgs a)
                     s_w64[jj] = make_double4(a.wx[j], a.wy[j], a.wz[j], a.u[j]);
                 } else {
                     const float hx = (float)jx, hy = (float)jy, hz = (float)jz;
+                    const double hj = P.Hs[j];
                     s_p32[jj] = make_float4(hx, hy, hz, (float)P.m[j]);
                     s_w32[jj] = make_float4((float)a.wx[j], (float)a.wy[j], (float)a.wz[j],
                                             (float)a.u[j]);
                     s_lo32[jj] = make_float4((float)(jx - (double)hx), (float)(jy - (double)hy),
-                                             (float)(jz - (double)hz), 0.0f);
+                                             (float)(jz - (double)hz), (float)(hj * hj));
                 }
             }
             __syncthreads();
-            if (!far || !active) continue;
-            if (FP64)
-                adj_stage_fp64(s_p64, s_w64, rx, ry, rz, mk, wkx, wky, wkz, uk, acc);
-            else if (wide)
-                adj_stage_fp32<true>(s_p32, s_w32, s_lo32, q, acc);
-            else
-                adj_stage_fp32<false>(s_p32, s_w32, s_lo32, q, acc);
+            if (!active) continue;
+            for (int h = 0; h < ATS / TC; ++h) {
+                const int c = st * (ATS / TC) + h;
+                const int o = h * TC;
+                if ((mask >> c) & 1u) {
+                    if (!FP64)
+                        adj_near_chunk_fp32<NK>(s_p32 + o, s_w32 + o, s_lo32 + o, q, a,
+                                                (int64_t)t * TJ + c * TC, i, Ox, Oy, Oz, acc);
+                    continue;
+                }
+                if (FP64)
+                    adj_chunk_fp64(s_p64 + o, s_w64 + o, rx, ry, rz, mk, wkx, wky, wkz, uk, acc);
+                else if (wide)
+                    adj_chunk_fp32<true>(s_p32 + o, s_w32 + o, s_lo32 + o, q, acc);
+                else
+                    adj_chunk_fp32<false>(s_p32 + o, s_w32 + o, s_lo32 + o, q, acc);
+            }
         }
     }
 
@@ -359,10 +451,10 @@ __global__ __launch_bounds__(WG) void k_adj_far(AdjArgs a)
         part[a.n + i] = mk * acc[1] - wky * acc[3] + acc[5];
         part[2 * a.n + i] = mk * acc[2] - wkz * acc[3] + acc[6];
         part[3 * a.n + i] = -acc[7];
-        part[4 * a.n + i] = 0.0;
+        part[4 * a.n + i] = acc[8];
     }
     if (a.stats && lane == 0) {
-        atomicAdd(&a.stats[0], (unsigned long long)(t1 - t0));
+        atomicAdd(&a.stats[0], (unsigned long long)(t1 - t0) * CHUNKS);
         atomicAdd(&a.stats[1], (unsigned long long)nearv);
     }
 }
@@ -410,9 +502,11 @@ __global__ __launch_bounds__(64 * ANEAR_WAVES) void k_adj_near(AdjArgs a)
     Acc9 g = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int cnt = 0;
     for (int t = t0; t < t1; ++t) {
-        if (!flags[t]) continue;
+        const unsigned fl = flags[t];
+        if (!fl) continue;
         if ((cnt++ % ANEAR_WAVES) != w) continue;
         for (int c0 = 0; c0 < TJ; c0 += ACHUNK) {
+            if (!((fl >> (c0 / TC)) & 1u)) continue;   // a far chunk of a near tile
             const int64_t j = (int64_t)t * TJ + c0 + lane;
             wave_lds_sync();
             lds.p[w][lane] = make_double4(P.x[j] - Ox, P.y[j] - Oy, P.z[j] - Oz, P.m[j]);
@@ -466,50 +560,44 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a)
 
 bool adj_config_supported(int wg) { return wg == 64 || wg == 128 || wg == 256; }
 
+template <int WG>
+static hipError_t adj_far_launch(int precision, const AdjArgs &a, hipStream_t s)
+{
+    const dim3 grid((unsigned)a.niblocks * (unsigned)a.jsplit), block(WG);
+    if (precision != 0) {
+        hipLaunchKernelGGL((k_adj_far<WG, true, 0>), grid, block, 0, s, a);
+    } else {
+        switch (a.nkernel) {
+        case 0: hipLaunchKernelGGL((k_adj_far<WG, false, 0>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_adj_far<WG, false, 1>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_adj_far<WG, false, 2>), grid, block, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_adj_far(int precision, const AdjArgs &a, int wg, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
-    const dim3 grid((unsigned)a.niblocks * (unsigned)a.jsplit);
-    const bool fp64 = precision != 0;
     switch (wg) {
-    case 64:
-        if (fp64) hipLaunchKernelGGL((k_adj_far<64, true>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((k_adj_far<64, false>), grid, dim3(64), 0, s, a);
-        break;
-    case 128:
-        if (fp64) hipLaunchKernelGGL((k_adj_far<128, true>), grid, dim3(128), 0, s, a);
-        else hipLaunchKernelGGL((k_adj_far<128, false>), grid, dim3(128), 0, s, a);
-        break;
-    case 256:
-        if (fp64) hipLaunchKernelGGL((k_adj_far<256, true>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_adj_far<256, false>), grid, dim3(256), 0, s, a);
-        break;
-    default:
-        return hipErrorInvalidConfiguration;
+    case 64: return adj_far_launch<64>(precision, a, s);
+    case 128: return adj_far_launch<128>(precision, a, s);
+    case 256: return adj_far_launch<256>(precision, a, s);
+    default: return hipErrorInvalidConfiguration;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_adj_near(int precision, const AdjArgs &a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
+    if (precision == 0) return hipSuccess;   // the FP32C far kernel took the near chunks
     const dim3 grid((unsigned)(a.ngroups * a.jsplit)), block(64 * ANEAR_WAVES);
-    const bool fast = precision == 0;
     switch (a.nkernel) {
-    case 0:
-        if (fast) hipLaunchKernelGGL((k_adj_near<true, 0>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_adj_near<false, 0>), grid, block, 0, s, a);
-        break;
-    case 1:
-        if (fast) hipLaunchKernelGGL((k_adj_near<true, 1>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_adj_near<false, 1>), grid, block, 0, s, a);
-        break;
-    case 2:
-        if (fast) hipLaunchKernelGGL((k_adj_near<true, 2>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_adj_near<false, 2>), grid, block, 0, s, a);
-        break;
-    default:
-        return hipErrorInvalidValue;
+    case 0: hipLaunchKernelGGL((k_adj_near<false, 0>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_adj_near<false, 1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_adj_near<false, 2>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -643,7 +643,7 @@ int sg_compute_batch_device(sg_context *c, int nsets, const long long *offsets, 
     a.az = az;
     a.pot = pot;
     a.stats = c->stats;
-    const int nwg = sg::batch_resident_workgroups(precision, wg, ipl, c->device);
+    const int nwg = sg::batch_resident_workgroups(precision, nkernel, wg, ipl, c->device);
     if (nwg <= 0) return SG_ERR_CONFIG;
 
     HIP_TRY(hipEventRecord(c->ev_k0, s));
@@ -844,6 +844,7 @@ int sg_vjp_device(sg_context *c, int n, const double *x, const double *y, const 
     a.wz = c->scot[2];
     a.u = c->scot[3];
     a.tiles = b.tiles;
+    a.chunks = b.chunks;
     a.n = n;
     a.nkernel = nkernel;
     a.ntiles = ntiles;

@@ -57,6 +57,24 @@ __device__ __forceinline__ bool boxes_far(float d2, float hmax, const TileMeta &
     return d2 > hm * hm * (1.0f + FAR_MARGIN);
 }
 
+// Near/far decision of one (wave, tile): 0 when every pair of the tile is
+// Newtonian for the wave, else bit c set for each chunk c of the tile whose
+// own box test fails (the near chunks).
+__device__ __forceinline__ unsigned near_mask(const TileMeta *chunks, int t, const TileMeta &tm,
+                                              float d2, double Ox, double Oy, double Oz,
+                                              const float (&bl)[3], const float (&bh)[3],
+                                              float hmax)
+{
+    if (uniform_i(boxes_far(d2, hmax, tm)) != 0) return 0u;
+    unsigned mask = 0;
+    for (int c = 0; c < CHUNKS; ++c) {
+        const TileMeta &cm = chunks[(int64_t)t * CHUNKS + c];
+        if (uniform_i(boxes_far(box_dist2(cm, Ox, Oy, Oz, bl, bh), hmax, cm)) == 0)
+            mask |= 1u << c;
+    }
+    return mask;
+}
+
 // Kernel polynomials A(q) (force) and P(q) (potential), q = r/H, with the
 // reference's decimal coefficients (cpu_grav.f:91-117, 210-243, 321-350).
 __device__ __forceinline__ void poly_AP(int nk, double q, double invq, double &A, double &P)

@@ -143,25 +143,153 @@ __device__ __forceinline__ void far_chunk_fp32(const float4 *s_hi, const float4 
     }
 }
 
-// Near/far decision of one (wave, tile): 0 when every pair of the tile is
-// Newtonian for the wave, else bit c set for each chunk c of the tile whose
-// own box test fails (those chunks go to the near kernel).
-__device__ __forceinline__ unsigned near_mask(const TileMeta *chunks, int t, const TileMeta &tm,
-                                              float d2, double Ox, double Oy, double Oz,
-                                              const float (&bl)[3], const float (&bh)[3],
-                                              float hmax)
+// One side k of a softened pair: adds F(r,H_k) and P(r,H_k) (without m_j).
+// The kernel choice NK is a template parameter of the near kernels: with one
+// polynomial per instantiation the register footprint drops by a third.
+// With PSI it also adds P(q) + q^2 A(q) to *ps when the side is softened: the
+// grad-h term's dphidh(2q) = -(P(q) + q^2 A(q))/2 (kernels.f:215-279).
+template <int NK, bool PSI = false>
+__device__ __forceinline__ void soft_side(double r, double ri, double r2, double H,
+                                          double invH, double &gacc, double &gpot,
+                                          double *ps = nullptr)
 {
-    if (uniform_i(boxes_far(d2, hmax, tm)) != 0) return 0u;
-    unsigned mask = 0;
-    for (int c = 0; c < CHUNKS; ++c) {
-        const TileMeta &cm = chunks[(int64_t)t * CHUNKS + c];
-        if (uniform_i(boxes_far(box_dist2(cm, Ox, Oy, Oz, bl, bh), hmax, cm)) == 0)
-            mask |= 1u << c;
+    if (r2 <= H * H) {
+        const double q = r * invH;
+        double A, P;
+        poly_AP(NK, q, H * ri, A, P);
+        gacc += A * invH * invH * invH;
+        gpot += P * invH;
+        if (PSI) *ps = fma(q * q, A, P);
+    } else {
+        gacc += ri * ri * ri;
+        gpot -= ri;
     }
-    return mask;
 }
 
-template <int WG, int IPL>
+// Ordered pair (i <- j), FP64, the semantics of grav_force_direct.cu:55-396
+// with the r == 0 rule documented in include/sphgrav.h.
+// With PSI, ps accumulates m_j (P(q_i) + q_i^2 A(q_i)) over the pairs with
+// r <= H_i (the grad-h softening sum before its -2/H_i^2 factor).
+// The j side's 1/H_j and H_j^2 come precomputed from the LDS stage (one
+// division per staged particle instead of one per pair).
+template <bool FAST, int NK, bool PSI = false>
+__device__ __forceinline__ void near_pair(double dx, double dy, double dz, double Hi,
+                                          double invHi, double hi2, double Hj, double invHj,
+                                          double hj2, double mj, bool gasj, bool same,
+                                          double &ax, double &ay, double &az, double &ap,
+                                          double *ps = nullptr)
+{
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    if (r2 >= fmax(hi2, hj2)) {
+        const double ri = rsqrt64<FAST>(r2);
+        const double mr = mj * ri;
+        const double mr3 = mr * ri * ri;
+        ax = fma(mr3, dx, ax);
+        ay = fma(mr3, dy, ay);
+        az = fma(mr3, dz, az);
+        ap -= mr;
+    } else if (r2 == 0.0) {
+        if (!same || gasj) ap -= self_coeff(NK) * mj * (invHi + invHj);
+        if (PSI) *ps -= 2.0 * self_coeff(NK) * mj;   // P(0) = -2c
+    } else {
+        const double ri = rsqrt64<FAST>(r2);
+        const double r = r2 * ri;
+        double gacc = 0.0, gpot = 0.0;
+        double psk = 0.0;
+        soft_side<NK, PSI>(r, ri, r2, Hi, invHi, gacc, gpot, &psk);
+        soft_side<NK>(r, ri, r2, Hj, invHj, gacc, gpot);
+        if (PSI) *ps = fma(mj, psk, *ps);
+        const double hm = 0.5 * mj;
+        const double f = hm * gacc;
+        ax = fma(f, dx, ax);
+        ay = fma(f, dy, ay);
+        az = fma(f, dz, az);
+        ap = fma(hm, gpot, ap);
+    }
+}
+
+// One near chunk (TC j) inside the far kernel.  Every pair runs the FP32
+// body with the lanes whose pair may be softened (r^2 < max(H_i^2, H_j^2) in
+// FP32, r = 0 included) masked out; the j where any lane was masked are
+// recorded, and afterwards those lanes' pairs run in FP64 from the sorted
+// arrays through near_pair (a pair is taken by exactly one of the two, so a
+// borderline FP32 decision costs nothing but rounding).
+template <int IPL, int NK>
+__device__ __forceinline__ void near_chunk_fp32(
+    const float4 *s_hi, const float4 *s_lo, const SortedSet &I, const SortedSet &J,
+    int64_t jbase, double Ox, double Oy, double Oz, const float (&xh)[IPL],
+    const float (&yh)[IPL], const float (&zh)[IPL], const float (&xl)[IPL],
+    const float (&yl)[IPL], const float (&zl)[IPL], const float (&hi2f)[IPL],
+    const int64_t (&ii)[IPL], double (&ax)[IPL], double (&ay)[IPL], double (&az)[IPL],
+    double (&ap)[IPL])
+{
+    uint64_t jmask = 0;   // wave-uniform: j with at least one masked lane
+#pragma unroll 1
+    for (int j0 = 0; j0 < TC; j0 += FOLD) {
+        float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
+        for (int jj = j0; jj < j0 + FOLD; ++jj) {
+            const float4 p = s_hi[jj];
+            const float4 q = s_lo[jj];   // q.w = H_j^2
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                const float dx = (p.x - xh[k]) + (q.x - xl[k]);
+                const float dy = (p.y - yh[k]) + (q.y - yl[k]);
+                const float dz = (p.z - zh[k]) + (q.z - zl[k]);
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const bool soft = r2 < fmaxf(hi2f[k], q.w);
+                const float ri = soft ? 0.0f : __builtin_amdgcn_rsqf(r2);
+                const float mr = p.w * ri;
+                const float mr3 = mr * ri * ri;
+                fx[k] = fmaf(mr3, dx, fx[k]);
+                fy[k] = fmaf(mr3, dy, fy[k]);
+                fz[k] = fmaf(mr3, dz, fz[k]);
+                fp[k] += mr;
+                any = any || soft;
+            }
+            if (__ballot(any)) jmask |= 1ull << jj;
+        }
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            ax[k] += (double)fx[k];
+            ay[k] += (double)fy[k];
+            az[k] += (double)fz[k];
+            ap[k] -= (double)fp[k];
+        }
+    }
+    while (jmask) {
+        const int jj = __builtin_ctzll(jmask);
+        jmask &= jmask - 1;
+        const float4 p = s_hi[jj];
+        const float4 q = s_lo[jj];
+        const int64_t j = jbase + jj;
+        const double jx = J.x[j] - Ox, jy = J.y[j] - Oy, jz = J.z[j] - Oz, mj = J.m[j];
+        const double hs = J.Hs[j], ha = fabs(hs);
+        const int idj = J.idx[j];
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            // the same FP32 decision as above
+            const float dx = (p.x - xh[k]) + (q.x - xl[k]);
+            const float dy = (p.y - yh[k]) + (q.y - yl[k]);
+            const float dz = (p.z - zh[k]) + (q.z - zl[k]);
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            if (r2 < fmaxf(hi2f[k], q.w)) {
+                const double Hi = fabs(I.Hs[ii[k]]);
+                near_pair<true, NK>(jx - (I.x[ii[k]] - Ox), jy - (I.y[ii[k]] - Oy),
+                                    jz - (I.z[ii[k]] - Oz), Hi, 1.0 / Hi, Hi * Hi, ha, 1.0 / ha,
+                                    ha * ha, mj, hs > 0.0, idj == I.idx[ii[k]], ax[k], ay[k],
+                                    az[k], ap[k]);
+            }
+        }
+    }
+}
+
+// FUSED: the chunks a wave cannot classify as far run near_chunk_fp32 here
+// (the forward FP32C path, no near kernel); otherwise they are flagged in
+// b.flags for a near kernel (the grad-h pass).
+template <int WG, int IPL, int NK, bool FUSED>
 __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const SortedSet &J,
                                                 const TileMeta *tiles, const TileMeta *chunks,
                                                 const Block &b, unsigned long long *stats,
@@ -175,7 +303,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
     const int64_t io = min(b.ibase + GI / 2, b.iend - 1);
     const double Ox = I.x[io], Oy = I.y[io], Oz = I.z[io];
 
-    float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL];
+    float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL], hi2f[IPL];
     int64_t ii[IPL];
     bool valid[IPL];
     float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -192,7 +320,9 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         xl[k] = (float)(rx - (double)xh[k]);
         yl[k] = (float)(ry - (double)yh[k]);
         zl[k] = (float)(rz - (double)zh[k]);
-        const float H = (float)fabs(I.Hs[ii[k]]);
+        const double Hd = fabs(I.Hs[ii[k]]);
+        hi2f[k] = (float)(Hd * Hd);
+        const float H = (float)Hd;
         bl[0] = fminf(bl[0], xh[k]); bl[1] = fminf(bl[1], yh[k]); bl[2] = fminf(bl[2], zh[k]);
         bh[0] = fmaxf(bh[0], xh[k]); bh[1] = fmaxf(bh[1], yh[k]); bh[2] = fmaxf(bh[2], zh[k]);
         hmax = fmaxf(hmax, H);
@@ -223,9 +353,9 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         const unsigned mask =
             active ? near_mask(chunks, t, tm, d2, Ox, Oy, Oz, bl, bh, hmax) : 0u;
         const bool wide = uniform_i(d2 < wide_d2) != 0;
-        if (lane == 0) flags[t - b.t0] = (unsigned char)mask;
+        if (!FUSED && lane == 0) flags[t - b.t0] = (unsigned char)mask;
         nearv += __builtin_popcount(mask);
-        if (WG == 64 && mask == (1u << CHUNKS) - 1) continue;
+        if (!FUSED && WG == 64 && mask == (1u << CHUNKS) - 1) continue;
         for (int s = 0; s < TJ / TS; ++s) {
             __syncthreads();
             for (int jj = tid; jj < TS; jj += WG) {
@@ -233,14 +363,25 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
                 const double rx = J.x[j] - Ox, ry = J.y[j] - Oy, rz = J.z[j] - Oz;
                 const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
                 s_hi[jj] = make_float4(hx, hy, hz, (float)J.m[j]);
+                float hj2 = 0.0f;
+                if (FUSED) {
+                    const double hj = J.Hs[j];
+                    hj2 = (float)(hj * hj);
+                }
                 s_lo[jj] = make_float4((float)(rx - (double)hx), (float)(ry - (double)hy),
-                                       (float)(rz - (double)hz), 0.0f);
+                                       (float)(rz - (double)hz), hj2);
             }
             __syncthreads();
             if (!active) continue;
             for (int h = 0; h < TS / TC; ++h) {
                 const int c = s * (TS / TC) + h;
-                if ((mask >> c) & 1u) continue;
+                if ((mask >> c) & 1u) {
+                    if (FUSED)
+                        near_chunk_fp32<IPL, NK>(s_hi + h * TC, s_lo + h * TC, I, J,
+                                                 (int64_t)t * TJ + c * TC, Ox, Oy, Oz, xh, yh, zh,
+                                                 xl, yl, zl, hi2f, ii, ax, ay, az, ap);
+                    continue;
+                }
                 if (wide)
                     far_chunk_fp32<IPL, true>(s_hi + h * TC, s_lo + h * TC, xh, yh, zh, xl, yl,
                                               zl, ax, ay, az, ap);
@@ -362,71 +503,6 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
     if (stats && lane == 0) {
         atomicAdd(&stats[0], (unsigned long long)(b.t1 - b.t0) * CHUNKS);
         atomicAdd(&stats[1], (unsigned long long)nearv);
-    }
-}
-
-// One side k of a softened pair: adds F(r,H_k) and P(r,H_k) (without m_j).
-// The kernel choice NK is a template parameter of the near kernels: with one
-// polynomial per instantiation the register footprint drops by a third.
-// With PSI it also adds P(q) + q^2 A(q) to *ps when the side is softened: the
-// grad-h term's dphidh(2q) = -(P(q) + q^2 A(q))/2 (kernels.f:215-279).
-template <int NK, bool PSI = false>
-__device__ __forceinline__ void soft_side(double r, double ri, double r2, double H,
-                                          double invH, double &gacc, double &gpot,
-                                          double *ps = nullptr)
-{
-    if (r2 <= H * H) {
-        const double q = r * invH;
-        double A, P;
-        poly_AP(NK, q, H * ri, A, P);
-        gacc += A * invH * invH * invH;
-        gpot += P * invH;
-        if (PSI) *ps = fma(q * q, A, P);
-    } else {
-        gacc += ri * ri * ri;
-        gpot -= ri;
-    }
-}
-
-// Ordered pair (i <- j), FP64, the semantics of grav_force_direct.cu:55-396
-// with the r == 0 rule documented in include/sphgrav.h.
-// With PSI, ps accumulates m_j (P(q_i) + q_i^2 A(q_i)) over the pairs with
-// r <= H_i (the grad-h softening sum before its -2/H_i^2 factor).
-// The j side's 1/H_j and H_j^2 come precomputed from the LDS stage (one
-// division per staged particle instead of one per pair).
-template <bool FAST, int NK, bool PSI = false>
-__device__ __forceinline__ void near_pair(double dx, double dy, double dz, double Hi,
-                                          double invHi, double hi2, double Hj, double invHj,
-                                          double hj2, double mj, bool gasj, bool same,
-                                          double &ax, double &ay, double &az, double &ap,
-                                          double *ps = nullptr)
-{
-    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    if (r2 >= fmax(hi2, hj2)) {
-        const double ri = rsqrt64<FAST>(r2);
-        const double mr = mj * ri;
-        const double mr3 = mr * ri * ri;
-        ax = fma(mr3, dx, ax);
-        ay = fma(mr3, dy, ay);
-        az = fma(mr3, dz, az);
-        ap -= mr;
-    } else if (r2 == 0.0) {
-        if (!same || gasj) ap -= self_coeff(NK) * mj * (invHi + invHj);
-        if (PSI) *ps -= 2.0 * self_coeff(NK) * mj;   // P(0) = -2c
-    } else {
-        const double ri = rsqrt64<FAST>(r2);
-        const double r = r2 * ri;
-        double gacc = 0.0, gpot = 0.0;
-        double psk = 0.0;
-        soft_side<NK, PSI>(r, ri, r2, Hi, invHi, gacc, gpot, &psk);
-        soft_side<NK>(r, ri, r2, Hj, invHj, gacc, gpot);
-        if (PSI) *ps = fma(mj, psk, *ps);
-        const double hm = 0.5 * mj;
-        const double f = hm * gacc;
-        ax = fma(f, dx, ax);
-        ay = fma(f, dy, ay);
-        az = fma(f, dz, az);
-        ap = fma(hm, gpot, ap);
     }
 }
 
@@ -639,13 +715,28 @@ __device__ __forceinline__ Block single_block(const PairArgs &a)
     return b;
 }
 
-template <int WG, int IPL>
-__global__ __launch_bounds__(WG) void k_far_fp32c(PairArgs a)
+// The forward FP32C pass: far and near chunks in one kernel.  With one i per
+// lane the register budget is held to 7 waves per SIMD, the occupancy of the
+// far loop alone; the near chunks' FP64 fallback then spills a few values
+// to scratch outside the pair loops.
+template <int WG, int IPL, int NK>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(IPL == 1 ? 7 : 1)))
+void k_far_fp32c(PairArgs a)
 {
     __shared__ float4 s_hi[TS];
     __shared__ float4 s_lo[TS];
-    far_block_fp32c<WG, IPL>(a.i, a.j, a.tiles, a.chunks, single_block<WG, IPL>(a), a.stats, s_hi,
-                             s_lo);
+    far_block_fp32c<WG, IPL, NK, true>(a.i, a.j, a.tiles, a.chunks, single_block<WG, IPL>(a),
+                                       a.stats, s_hi, s_lo);
+}
+
+// Far chunks only, near chunks flagged (the pass with the grad-h term).
+template <int WG>
+__global__ __launch_bounds__(WG) void k_far_fp32c_split(PairArgs a)
+{
+    __shared__ float4 s_hi[TS];
+    __shared__ float4 s_lo[TS];
+    far_block_fp32c<WG, 1, 0, false>(a.i, a.j, a.tiles, a.chunks, single_block<WG, 1>(a),
+                                     a.stats, s_hi, s_lo);
 }
 
 template <int WG, int IPL>
@@ -794,8 +885,9 @@ __device__ __forceinline__ Block batch_block(const BatchArgs &a, int item)
 // Persistent far kernel over a batch: every workgroup takes the next
 // (set, i-block) item from the queue until the queue is empty, so every
 // workgroup reaches the exit once all items are taken.
-template <int WG, int IPL, bool FP64>
-__global__ __launch_bounds__(WG) void k_far_batch(BatchArgs a)
+template <int WG, int IPL, bool FP64, int NK>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(IPL == 1 && !FP64 ? 7 : 1)))
+void k_far_batch(BatchArgs a)
 {
     // one TS-entry FP64 tile, or the two TS-entry FP32 tiles (hi, lo)
     __shared__ double4 s_buf[TS];
@@ -812,7 +904,8 @@ __global__ __launch_bounds__(WG) void k_far_batch(BatchArgs a)
         if (FP64)
             far_block_fp64<WG, IPL>(a.j, a.j, a.tiles, a.chunks, b, a.stats, s_buf);
         else
-            far_block_fp32c<WG, IPL>(a.j, a.j, a.tiles, a.chunks, b, a.stats, s_hi, s_lo);
+            far_block_fp32c<WG, IPL, NK, true>(a.j, a.j, a.tiles, a.chunks, b, a.stats, s_hi,
+                                               s_lo);
     }
 }
 
@@ -866,13 +959,40 @@ inline dim3 grid_of(const PairArgs &a) { return dim3((unsigned)a.niblocks * (uns
         return hipGetLastError();                                                      \
     }
 
+template <int WG, int IPL>
+static hipError_t far_fused_launch(const PairArgs &a, hipStream_t s)
+{
+    switch (a.nkernel) {
+    case 0: hipLaunchKernelGGL((k_far_fp32c<WG, IPL, 0>), grid_of(a), dim3(WG), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_far_fp32c<WG, IPL, 1>), grid_of(a), dim3(WG), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_far_fp32c<WG, IPL, 2>), grid_of(a), dim3(WG), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// FP32C: the fused far kernel (no near kernel follows), or, with the grad-h
+// term, the split far kernel with one i per lane.  FP64: the far kernel.
 hipError_t launch_pair_far(int precision, const PairArgs &a, int wg, hipStream_t s)
 {
     if (a.ni <= 0) return hipSuccess;
+    if (precision == 0 && a.psi) {
+        if (a.ipl != 1) return hipErrorInvalidConfiguration;
+        switch (wg) {
+        case 64: hipLaunchKernelGGL((k_far_fp32c_split<64>), grid_of(a), dim3(64), 0, s, a); break;
+        case 128: hipLaunchKernelGGL((k_far_fp32c_split<128>), grid_of(a), dim3(128), 0, s, a); break;
+        case 256: hipLaunchKernelGGL((k_far_fp32c_split<256>), grid_of(a), dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidConfiguration;
+        }
+        return hipGetLastError();
+    }
     if (precision == 0) {
-        SG_FAR(k_far_fp32c, 64, 1) SG_FAR(k_far_fp32c, 64, 2) SG_FAR(k_far_fp32c, 64, 4)
-        SG_FAR(k_far_fp32c, 128, 1) SG_FAR(k_far_fp32c, 128, 2) SG_FAR(k_far_fp32c, 128, 4)
-        SG_FAR(k_far_fp32c, 256, 1) SG_FAR(k_far_fp32c, 256, 2) SG_FAR(k_far_fp32c, 256, 4)
+        if (wg == 64 && a.ipl == 1) return far_fused_launch<64, 1>(a, s);
+        if (wg == 64 && a.ipl == 2) return far_fused_launch<64, 2>(a, s);
+        if (wg == 128 && a.ipl == 1) return far_fused_launch<128, 1>(a, s);
+        if (wg == 128 && a.ipl == 2) return far_fused_launch<128, 2>(a, s);
+        if (wg == 256 && a.ipl == 1) return far_fused_launch<256, 1>(a, s);
+        if (wg == 256 && a.ipl == 2) return far_fused_launch<256, 2>(a, s);
     } else {
         SG_FAR(k_far_fp64, 64, 1) SG_FAR(k_far_fp64, 64, 2)
         SG_FAR(k_far_fp64, 128, 1) SG_FAR(k_far_fp64, 128, 2)
@@ -881,13 +1001,13 @@ hipError_t launch_pair_far(int precision, const PairArgs &a, int wg, hipStream_t
     return hipErrorInvalidConfiguration;
 }
 
-template <int IPL, bool FAST>
+template <int IPL>
 static void near_launch(const PairArgs &a, dim3 grid, dim3 block, hipStream_t s)
 {
     switch (a.nkernel) {
-    case 0: hipLaunchKernelGGL((k_near<IPL, FAST, 0>), grid, block, 0, s, a); break;
-    case 1: hipLaunchKernelGGL((k_near<IPL, FAST, 1>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_near<IPL, FAST, 2>), grid, block, 0, s, a); break;
+    case 0: hipLaunchKernelGGL((k_near<IPL, false, 0>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_near<IPL, false, 1>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_near<IPL, false, 2>), grid, block, 0, s, a); break;
     }
 }
 
@@ -901,6 +1021,8 @@ static void near_psi_launch(const PairArgs &a, dim3 grid, dim3 block, hipStream_
     }
 }
 
+// The near kernel of the FP64 pass and of the grad-h pass; the FP32C
+// forward pass has none (its far kernel takes the near chunks).
 hipError_t launch_pair_near(int precision, const PairArgs &a, hipStream_t s)
 {
     if (a.ni <= 0) return hipSuccess;
@@ -915,28 +1037,19 @@ hipError_t launch_pair_near(int precision, const PairArgs &a, hipStream_t s)
         else near_psi_launch<false>(a, grid, block, s);
         return hipGetLastError();
     }
-    if (precision == 0) {
-        switch (a.ipl) {
-        case 1: near_launch<1, true>(a, grid, block, s); break;
-        case 2: near_launch<2, true>(a, grid, block, s); break;
-        case 4: near_launch<4, true>(a, grid, block, s); break;
-        default: return hipErrorInvalidConfiguration;
-        }
-    } else {
-        switch (a.ipl) {
-        case 1: near_launch<1, false>(a, grid, block, s); break;
-        case 2: near_launch<2, false>(a, grid, block, s); break;
-        default: return hipErrorInvalidConfiguration;
-        }
+    if (precision == 0) return hipSuccess;
+    switch (a.ipl) {
+    case 1: near_launch<1>(a, grid, block, s); break;
+    case 2: near_launch<2>(a, grid, block, s); break;
+    default: return hipErrorInvalidConfiguration;
     }
     return hipGetLastError();
 }
 
 bool pair_config_supported(int precision, int wg, int ipl)
 {
-    const bool wg_ok = wg == 64 || wg == 128 || wg == 256;
-    if (precision == 0) return wg_ok && (ipl == 1 || ipl == 2 || ipl == 4);
-    return wg_ok && (ipl == 1 || ipl == 2);
+    (void)precision;
+    return (wg == 64 || wg == 128 || wg == 256) && (ipl == 1 || ipl == 2);
 }
 
 hipError_t launch_psi_only(const PairArgs &a, hipStream_t s)
@@ -960,45 +1073,63 @@ hipError_t launch_reduce_partials(const PairArgs &a, hipStream_t s)
 }
 
 // ------------------------------------------------------------ batch launches
-template <int WG, int IPL, bool FP64>
+// FP32C batches run the fused far kernel (one instantiation per softening
+// kernel NK); FP64 batches the plain far kernel (NK = 0) and then the near
+// kernel.
+template <int WG, int IPL, bool FP64, int NK>
 static hipError_t far_batch_launch(const BatchArgs &a, int nwg, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_far_batch<WG, IPL, FP64>), dim3(nwg), dim3(WG), 0, s, a);
+    hipLaunchKernelGGL((k_far_batch<WG, IPL, FP64, NK>), dim3(nwg), dim3(WG), 0, s, a);
     return hipGetLastError();
 }
 
-template <int WG, int IPL, bool FP64>
+template <int WG, int IPL, bool FP64, int NK>
 static int far_batch_resident(int device)
 {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_far_batch<WG, IPL, FP64>, WG, 0) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_far_batch<WG, IPL, FP64, NK>, WG,
+                                                     0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return 0;
     return per_cu * cus;
 }
 
-#define SG_FARB(WG, IPL)                                                                     \
-    if (wg == WG && ipl == IPL) {                                                            \
-        if (resident_only)                                                                   \
-            return precision == 0 ? far_batch_resident<WG, IPL, false>(device)               \
-                                  : far_batch_resident<WG, IPL, true>(device);               \
-        return (int)(precision == 0 ? far_batch_launch<WG, IPL, false>(a, nwg, s)            \
-                                    : far_batch_launch<WG, IPL, true>(a, nwg, s));           \
+template <int WG, int IPL>
+static int far_batch_one(int precision, int nk, const BatchArgs &a, int nwg, hipStream_t s,
+                         bool resident_only, int device)
+{
+    if (precision != 0)
+        return resident_only ? far_batch_resident<WG, IPL, true, 0>(device)
+                             : (int)far_batch_launch<WG, IPL, true, 0>(a, nwg, s);
+    switch (nk) {
+    case 0:
+        return resident_only ? far_batch_resident<WG, IPL, false, 0>(device)
+                             : (int)far_batch_launch<WG, IPL, false, 0>(a, nwg, s);
+    case 1:
+        return resident_only ? far_batch_resident<WG, IPL, false, 1>(device)
+                             : (int)far_batch_launch<WG, IPL, false, 1>(a, nwg, s);
+    default:
+        return resident_only ? far_batch_resident<WG, IPL, false, 2>(device)
+                             : (int)far_batch_launch<WG, IPL, false, 2>(a, nwg, s);
     }
+}
 
-static int far_batch_dispatch(int precision, const BatchArgs &a, int wg, int ipl, int nwg,
+static int far_batch_dispatch(int precision, int nk, const BatchArgs &a, int wg, int ipl, int nwg,
                               hipStream_t s, bool resident_only, int device)
 {
-    SG_FARB(64, 1) SG_FARB(64, 2) SG_FARB(128, 1) SG_FARB(128, 2) SG_FARB(256, 1)
-    SG_FARB(256, 2)
+    if (wg == 64 && ipl == 1) return far_batch_one<64, 1>(precision, nk, a, nwg, s, resident_only, device);
+    if (wg == 64 && ipl == 2) return far_batch_one<64, 2>(precision, nk, a, nwg, s, resident_only, device);
+    if (wg == 128 && ipl == 1) return far_batch_one<128, 1>(precision, nk, a, nwg, s, resident_only, device);
+    if (wg == 128 && ipl == 2) return far_batch_one<128, 2>(precision, nk, a, nwg, s, resident_only, device);
+    if (wg == 256 && ipl == 1) return far_batch_one<256, 1>(precision, nk, a, nwg, s, resident_only, device);
+    if (wg == 256 && ipl == 2) return far_batch_one<256, 2>(precision, nk, a, nwg, s, resident_only, device);
     return resident_only ? 0 : (int)hipErrorInvalidConfiguration;
 }
 
-int batch_resident_workgroups(int precision, int wg, int ipl, int device)
+int batch_resident_workgroups(int precision, int nkernel, int wg, int ipl, int device)
 {
     BatchArgs dummy{};
-    return far_batch_dispatch(precision, dummy, wg, ipl, 0, nullptr, true, device);
+    return far_batch_dispatch(precision, nkernel, dummy, wg, ipl, 0, nullptr, true, device);
 }
 
 bool batch_config_supported(int wg, int ipl)
@@ -1006,13 +1137,13 @@ bool batch_config_supported(int wg, int ipl)
     return (wg == 64 || wg == 128 || wg == 256) && (ipl == 1 || ipl == 2);
 }
 
-template <int IPL, bool FAST>
+template <int IPL>
 static void near_batch_launch(const BatchArgs &a, dim3 grid, dim3 block, hipStream_t s)
 {
     switch (a.nkernel) {
-    case 0: hipLaunchKernelGGL((k_near_batch<IPL, FAST, 0>), grid, block, 0, s, a); break;
-    case 1: hipLaunchKernelGGL((k_near_batch<IPL, FAST, 1>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_near_batch<IPL, FAST, 2>), grid, block, 0, s, a); break;
+    case 0: hipLaunchKernelGGL((k_near_batch<IPL, false, 0>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_near_batch<IPL, false, 1>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_near_batch<IPL, false, 2>), grid, block, 0, s, a); break;
     }
 }
 
@@ -1022,17 +1153,12 @@ hipError_t launch_batch(int precision, const BatchArgs &a, int wg, int ipl, int 
     if (a.padded <= 0) return hipSuccess;
     hipError_t e = launch_fill_zero_int((int *)a.counter, 1, s);
     if (e != hipSuccess) return e;
-    e = (hipError_t)far_batch_dispatch(precision, a, wg, ipl, nwg, s, false, 0);
+    e = (hipError_t)far_batch_dispatch(precision, a.nkernel, a, wg, ipl, nwg, s, false, 0);
     if (e != hipSuccess) return e;
     const dim3 block(64 * NEAR_WAVES), grid((unsigned)a.ngroups);
-    if (a.ngroups > 0) {
-        if (precision == 0) {
-            if (ipl == 1) near_batch_launch<1, true>(a, grid, block, s);
-            else near_batch_launch<2, true>(a, grid, block, s);
-        } else {
-            if (ipl == 1) near_batch_launch<1, false>(a, grid, block, s);
-            else near_batch_launch<2, false>(a, grid, block, s);
-        }
+    if (precision != 0 && a.ngroups > 0) {
+        if (ipl == 1) near_batch_launch<1>(a, grid, block, s);
+        else near_batch_launch<2>(a, grid, block, s);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_reduce_batch, dim3((unsigned)((a.padded + 255) / 256)), dim3(256), 0, s,

@@ -95,6 +95,7 @@ struct AdjArgs {
     SortedSet p;                        // sorted particles (i and j)
     const double *wx, *wy, *wz, *u;     // sorted-order cotangents (0 on padding)
     const TileMeta *tiles;
+    const TileMeta *chunks;
     int64_t n;
     int nkernel;
     int ntiles;
@@ -124,7 +125,7 @@ bool pair_config_supported(int precision, int wg, int ipl);
 // Batch: queue reset, persistent far kernel on nwg workgroups, near kernel,
 // scatter.  Contains no memset, so it can be captured in a graph.
 hipError_t launch_batch(int precision, const BatchArgs &a, int wg, int ipl, int nwg, hipStream_t s);
-int batch_resident_workgroups(int precision, int wg, int ipl, int device);
+int batch_resident_workgroups(int precision, int nkernel, int wg, int ipl, int device);
 bool batch_config_supported(int wg, int ipl);
 
 // Preparation (sg_prep.hip).

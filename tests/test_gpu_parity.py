@@ -191,7 +191,7 @@ def test_deterministic(grav, plummer16k):
         np.testing.assert_array_equal(x, y)
 
 
-@pytest.mark.parametrize("prec,wg,ipl", [(SG_PREC_FP32C, w, i) for w in (64, 128, 256) for i in (1, 2, 4)]
+@pytest.mark.parametrize("prec,wg,ipl", [(SG_PREC_FP32C, w, i) for w in (64, 128, 256) for i in (1, 2)]
                          + [(SG_PREC_FP64, w, i) for w in (64, 128, 256) for i in (1, 2)])
 @pytest.mark.parametrize("jsplit", [0, 1, 3])
 def test_tile_configurations(grav, plummer16k, prec, wg, ipl, jsplit):

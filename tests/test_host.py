@@ -28,3 +28,12 @@ def test_profiled_traffic_reads_committed_profile():
     t = b.profiled_traffic(256, 1)
     assert t is not None and t > 1e9
     assert b.profiled_traffic(64, 4) is None
+
+
+def test_bench_refuses_several_gpus():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
