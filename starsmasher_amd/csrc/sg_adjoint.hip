@@ -391,6 +391,7 @@ void k_adj_far(AdjArgs a)
         reach = fmaxf(reach, fmaxf(-bl[c], bh[c]));
     }
     const float hmax = wave_max((float)Hd);
+    const float px[1] = {q.xh}, py[1] = {q.yh}, pz[1] = {q.zh}, ph[1] = {(float)Hd};
     const float wide_d2 = 3.0f * (WIDE_K * reach) * (WIDE_K * reach);
     const bool active = uniform_i(ibase + wv * 64 < iend) != 0;
     unsigned char *flags = a.nearflag + ((int64_t)ib * (WG / 64) + wv) * a.ntiles;
@@ -401,7 +402,8 @@ void k_adj_far(AdjArgs a)
         const TileMeta &tm = a.tiles[t];
         const float d2 = box_dist2(tm, Ox, Oy, Oz, bl, bh);
         const unsigned mask =
-            active ? near_mask(a.chunks, t, tm, d2, Ox, Oy, Oz, bl, bh, hmax) : 0u;
+            active ? near_mask<1>(a.chunks, t, tm, d2, Ox, Oy, Oz, bl, bh, hmax, px, py, pz, ph)
+                   : 0u;
         const bool wide = uniform_i(d2 < wide_d2) != 0;
         if (FP64 && lane == 0) flags[t] = (unsigned char)mask;
         nearv += __builtin_popcount(mask);

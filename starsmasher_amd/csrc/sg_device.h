@@ -58,19 +58,42 @@ __device__ __forceinline__ bool boxes_far(float d2, float hmax, const TileMeta &
 }
 
 // Near/far decision of one (wave, tile): 0 when every pair of the tile is
-// Newtonian for the wave, else bit c set for each chunk c of the tile whose
-// own box test fails (the near chunks).
+// Newtonian for the wave, else bit c set for each chunk c of the tile that
+// holds a pair that may be softened (the near chunks).  A chunk passes when
+// the wave's box is far from the chunk box, or else when every lane's own
+// i is farther from the chunk box than max(H_i, largest H of the chunk).
+// The per-lane test matters for waves whose space-filling-curve range
+// straddles a jump: their box spans much of the set, and a box test alone
+// would call most chunks near.
+template <int IPL>
 __device__ __forceinline__ unsigned near_mask(const TileMeta *chunks, int t, const TileMeta &tm,
                                               float d2, double Ox, double Oy, double Oz,
                                               const float (&bl)[3], const float (&bh)[3],
-                                              float hmax)
+                                              float hmax, const float (&px)[IPL],
+                                              const float (&py)[IPL], const float (&pz)[IPL],
+                                              const float (&ph)[IPL])
 {
     if (uniform_i(boxes_far(d2, hmax, tm)) != 0) return 0u;
     unsigned mask = 0;
     for (int c = 0; c < CHUNKS; ++c) {
         const TileMeta &cm = chunks[(int64_t)t * CHUNKS + c];
-        if (uniform_i(boxes_far(box_dist2(cm, Ox, Oy, Oz, bl, bh), hmax, cm)) == 0)
-            mask |= 1u << c;
+        if (uniform_i(boxes_far(box_dist2(cm, Ox, Oy, Oz, bl, bh), hmax, cm)) != 0) continue;
+        const float lx = (float)(cm.lo[0] - Ox), ly = (float)(cm.lo[1] - Oy),
+                    lz = (float)(cm.lo[2] - Oz);
+        const float ux = (float)(cm.hi[0] - Ox), uy = (float)(cm.hi[1] - Oy),
+                    uz = (float)(cm.hi[2] - Oz);
+        const float hc = (float)cm.maxH;
+        bool lane_near = false;
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            const float gx = fmaxf(0.0f, fmaxf(lx - px[k], px[k] - ux));
+            const float gy = fmaxf(0.0f, fmaxf(ly - py[k], py[k] - uy));
+            const float gz = fmaxf(0.0f, fmaxf(lz - pz[k], pz[k] - uz));
+            const float dl = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+            const float hm = fmaxf(ph[k], hc);
+            lane_near = lane_near || !(dl > hm * hm * (1.0f + FAR_MARGIN));
+        }
+        if (__ballot(lane_near)) mask |= 1u << c;
     }
     return mask;
 }

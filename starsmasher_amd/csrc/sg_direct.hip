@@ -208,21 +208,73 @@ __device__ __forceinline__ void near_pair(double dx, double dy, double dz, doubl
     }
 }
 
-// One near chunk (TC j) inside the far kernel.  Every pair runs the FP32
-// body with the lanes whose pair may be softened (r^2 < max(H_i^2, H_j^2) in
-// FP32, r = 0 included) masked out; the j where any lane was masked are
-// recorded, and afterwards those lanes' pairs run in FP64 from the sorted
-// arrays through near_pair (a pair is taken by exactly one of the two, so a
-// borderline FP32 decision costs nothing but rounding).
+// FP64 i-side state of the near pairs (positions relative to the origin).
+template <int IPL>
+__device__ __forceinline__ void near_i_state(const SortedSet &I, const int64_t (&ii)[IPL],
+                                             double Ox, double Oy, double Oz, double (&xr)[IPL],
+                                             double (&yr)[IPL], double (&zr)[IPL],
+                                             double (&Hi)[IPL], double (&invHi)[IPL],
+                                             double (&hi2)[IPL], int (&idi)[IPL])
+{
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        xr[k] = I.x[ii[k]] - Ox;
+        yr[k] = I.y[ii[k]] - Oy;
+        zr[k] = I.z[ii[k]] - Oz;
+        Hi[k] = fabs(I.Hs[ii[k]]);
+        invHi[k] = 1.0 / Hi[k];
+        hi2[k] = Hi[k] * Hi[k];
+        idi[k] = I.idx[ii[k]];
+    }
+}
+
+// Per-wave LDS of the fused FP32C far kernel: the wave's list of deferred
+// near tiles and the slice a near chunk is staged in.
+constexpr int NLIST = 128;   // deferred near tiles per wave (more run at once)
+struct NearSlice {
+    double4 p[TC];           // x, y, z relative to the origin; w: {FP32 m, FP32 H^2}
+    double m[TC];
+    double hs[TC];           // signed H (> 0: gas)
+    int idx[TC];
+    uint32_t list[NLIST];    // (tile - t0) << 4 | chunk mask
+};
+
+// Stages chunk c of tile t in the wave's slice and evaluates it.  Each pair
+// difference is formed in FP64 and rounded once to FP32 (two ops per axis;
+// near pairs are close, where the hi+lo difference would lose the low bits
+// of the i position whenever |x_i - O| < |x_j - O|, an error that does not
+// average out over j).  The FP32 body then runs every pair with the lanes
+// whose pair may be softened (r^2 < max(H_i^2, H_j^2) in FP32, r = 0
+// included) masked out; the j where a lane was masked are recorded in a
+// wave-uniform bit mask, and those lanes' pairs then run in FP64 through
+// near_pair from the slice (a pair is taken by exactly one of the two, so a
+// borderline FP32 decision costs only rounding).  Wave-local: no block
+// barrier.
 template <int IPL, int NK>
-__device__ __forceinline__ void near_chunk_fp32(
-    const float4 *s_hi, const float4 *s_lo, const SortedSet &I, const SortedSet &J,
-    int64_t jbase, double Ox, double Oy, double Oz, const float (&xh)[IPL],
-    const float (&yh)[IPL], const float (&zh)[IPL], const float (&xl)[IPL],
-    const float (&yl)[IPL], const float (&zl)[IPL], const float (&hi2f)[IPL],
-    const int64_t (&ii)[IPL], double (&ax)[IPL], double (&ay)[IPL], double (&az)[IPL],
+__device__ __forceinline__ void near_chunk_slice(
+    NearSlice &sl, const SortedSet &J, int t, int c, double Ox, double Oy, double Oz,
+    const double (&xr)[IPL], const double (&yr)[IPL], const double (&zr)[IPL],
+    const double (&Hi)[IPL], const double (&invHi)[IPL], const double (&hi2)[IPL],
+    const int (&idi)[IPL], double (&ax)[IPL], double (&ay)[IPL], double (&az)[IPL],
     double (&ap)[IPL])
 {
+    const int lane = threadIdx.x & 63;
+    {
+        const int64_t j = (int64_t)t * TJ + c * TC + lane;
+        const double rx = J.x[j] - Ox, ry = J.y[j] - Oy, rz = J.z[j] - Oz, mj = J.m[j];
+        const double hs = J.Hs[j];
+        const int id = J.idx[j];
+        const float mh[2] = {(float)mj, (float)(hs * hs)};
+        wave_lds_sync();
+        sl.p[lane] = make_double4(rx, ry, rz, __builtin_bit_cast(double, mh));
+        sl.m[lane] = mj;
+        sl.hs[lane] = hs;
+        sl.idx[lane] = id;
+        wave_lds_sync();
+    }
+    float hi2f[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) hi2f[k] = (float)hi2[k];
     uint64_t jmask = 0;   // wave-uniform: j with at least one masked lane
 #pragma unroll 1
     for (int j0 = 0; j0 < TC; j0 += FOLD) {
@@ -230,18 +282,18 @@ __device__ __forceinline__ void near_chunk_fp32(
 #pragma unroll
         for (int k = 0; k < IPL; ++k) fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
         for (int jj = j0; jj < j0 + FOLD; ++jj) {
-            const float4 p = s_hi[jj];
-            const float4 q = s_lo[jj];   // q.w = H_j^2
+            const double4 p = sl.p[jj];
+            const float2 mh = __builtin_bit_cast(float2, p.w);   // m_j, H_j^2
             bool any = false;
 #pragma unroll
             for (int k = 0; k < IPL; ++k) {
-                const float dx = (p.x - xh[k]) + (q.x - xl[k]);
-                const float dy = (p.y - yh[k]) + (q.y - yl[k]);
-                const float dz = (p.z - zh[k]) + (q.z - zl[k]);
+                const float dx = (float)(p.x - xr[k]);
+                const float dy = (float)(p.y - yr[k]);
+                const float dz = (float)(p.z - zr[k]);
                 const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const bool soft = r2 < fmaxf(hi2f[k], q.w);
+                const bool soft = r2 < fmaxf(hi2f[k], mh.y);
                 const float ri = soft ? 0.0f : __builtin_amdgcn_rsqf(r2);
-                const float mr = p.w * ri;
+                const float mr = mh.x * ri;
                 const float mr3 = mr * ri * ri;
                 fx[k] = fmaf(mr3, dx, fx[k]);
                 fy[k] = fmaf(mr3, dy, fy[k]);
@@ -262,38 +314,34 @@ __device__ __forceinline__ void near_chunk_fp32(
     while (jmask) {
         const int jj = __builtin_ctzll(jmask);
         jmask &= jmask - 1;
-        const float4 p = s_hi[jj];
-        const float4 q = s_lo[jj];
-        const int64_t j = jbase + jj;
-        const double jx = J.x[j] - Ox, jy = J.y[j] - Oy, jz = J.z[j] - Oz, mj = J.m[j];
-        const double hs = J.Hs[j], ha = fabs(hs);
-        const int idj = J.idx[j];
+        const double4 p = sl.p[jj];
+        const float2 mh = __builtin_bit_cast(float2, p.w);
+        const double mj = sl.m[jj], hs = sl.hs[jj], ha = fabs(hs);
+        const int idj = sl.idx[jj];
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
+            const double ddx = p.x - xr[k], ddy = p.y - yr[k], ddz = p.z - zr[k];
             // the same FP32 decision as above
-            const float dx = (p.x - xh[k]) + (q.x - xl[k]);
-            const float dy = (p.y - yh[k]) + (q.y - yl[k]);
-            const float dz = (p.z - zh[k]) + (q.z - zl[k]);
+            const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            if (r2 < fmaxf(hi2f[k], q.w)) {
-                const double Hi = fabs(I.Hs[ii[k]]);
-                near_pair<true, NK>(jx - (I.x[ii[k]] - Ox), jy - (I.y[ii[k]] - Oy),
-                                    jz - (I.z[ii[k]] - Oz), Hi, 1.0 / Hi, Hi * Hi, ha, 1.0 / ha,
-                                    ha * ha, mj, hs > 0.0, idj == I.idx[ii[k]], ax[k], ay[k],
-                                    az[k], ap[k]);
-            }
+            if (r2 < fmaxf(hi2f[k], mh.y))
+                near_pair<true, NK>(ddx, ddy, ddz, Hi[k], invHi[k], hi2[k], ha, 1.0 / ha, ha * ha,
+                                    mj, hs > 0.0, idj == idi[k], ax[k], ay[k], az[k], ap[k]);
         }
     }
 }
 
-// FUSED: the chunks a wave cannot classify as far run near_chunk_fp32 here
-// (the forward FP32C path, no near kernel); otherwise they are flagged in
-// b.flags for a near kernel (the grad-h pass).
+// FUSED (the forward FP32C pass): the chunks a wave cannot classify as far
+// are listed and run by the wave alone after its far loop (in its own LDS
+// slice, no block barrier in between, so one wave's near work does not hold
+// the others at the stage barriers); a list that fills up runs the rest at
+// once.  Not FUSED (the grad-h pass): they are flagged in b.flags for the
+// near kernel.
 template <int WG, int IPL, int NK, bool FUSED>
 __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const SortedSet &J,
                                                 const TileMeta *tiles, const TileMeta *chunks,
                                                 const Block &b, unsigned long long *stats,
-                                                float4 *s_hi, float4 *s_lo)
+                                                float4 *s_hi, float4 *s_lo, NearSlice *s_near)
 {
     constexpr int WI = 64 * IPL;   // i per wave
     constexpr int GI = WG * IPL;   // i per workgroup
@@ -303,7 +351,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
     const int64_t io = min(b.ibase + GI / 2, b.iend - 1);
     const double Ox = I.x[io], Oy = I.y[io], Oz = I.z[io];
 
-    float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL], hi2f[IPL];
+    float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL], hf[IPL];
     int64_t ii[IPL];
     bool valid[IPL];
     float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -320,12 +368,10 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         xl[k] = (float)(rx - (double)xh[k]);
         yl[k] = (float)(ry - (double)yh[k]);
         zl[k] = (float)(rz - (double)zh[k]);
-        const double Hd = fabs(I.Hs[ii[k]]);
-        hi2f[k] = (float)(Hd * Hd);
-        const float H = (float)Hd;
+        hf[k] = (float)fabs(I.Hs[ii[k]]);
         bl[0] = fminf(bl[0], xh[k]); bl[1] = fminf(bl[1], yh[k]); bl[2] = fminf(bl[2], zh[k]);
         bh[0] = fmaxf(bh[0], xh[k]); bh[1] = fmaxf(bh[1], yh[k]); bh[2] = fmaxf(bh[2], zh[k]);
-        hmax = fmaxf(hmax, H);
+        hmax = fmaxf(hmax, hf[k]);
     }
     float reach = 0.0f;   // max |x_i - O| over the wave, per axis
     for (int c = 0; c < 3; ++c) {
@@ -346,16 +392,37 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
     // computes nothing and flags nothing
     const bool active = uniform_i(b.ibase + wv * WI < b.iend) != 0;
     unsigned nearv = 0;
+    int nlist = 0;   // wave-uniform
     unsigned char *flags = b.flags + wv * b.fstride;
     for (int t = b.t0; t < b.t1; ++t) {
         const TileMeta &tm = tiles[t];
         const float d2 = box_dist2(tm, Ox, Oy, Oz, bl, bh);
         const unsigned mask =
-            active ? near_mask(chunks, t, tm, d2, Ox, Oy, Oz, bl, bh, hmax) : 0u;
+            active ? near_mask<IPL>(chunks, t, tm, d2, Ox, Oy, Oz, bl, bh, hmax, xh, yh, zh, hf)
+                   : 0u;
         const bool wide = uniform_i(d2 < wide_d2) != 0;
-        if (!FUSED && lane == 0) flags[t - b.t0] = (unsigned char)mask;
         nearv += __builtin_popcount(mask);
-        if (!FUSED && WG == 64 && mask == (1u << CHUNKS) - 1) continue;
+        bool near_tile_now = false;   // a full list: this tile's near chunks run below
+        if (!FUSED) {
+            if (lane == 0) flags[t - b.t0] = (unsigned char)mask;
+        } else if (mask) {
+            if (nlist < NLIST) {
+                if (lane == 0) s_near[wv].list[nlist] = ((uint32_t)(t - b.t0) << 4) | mask;
+                ++nlist;
+            } else {
+                near_tile_now = true;
+            }
+        }
+        if (FUSED && near_tile_now) {
+            double xr[IPL], yr[IPL], zr[IPL], Hi[IPL], invHi[IPL], hi2[IPL];
+            int idi[IPL];
+            near_i_state<IPL>(I, ii, Ox, Oy, Oz, xr, yr, zr, Hi, invHi, hi2, idi);
+            for (int c = 0; c < CHUNKS; ++c)
+                if ((mask >> c) & 1u)
+                    near_chunk_slice<IPL, NK>(s_near[wv], J, t, c, Ox, Oy, Oz, xr, yr, zr, Hi,
+                                              invHi, hi2, idi, ax, ay, az, ap);
+        }
+        if (WG == 64 && mask == (1u << CHUNKS) - 1) continue;
         for (int s = 0; s < TJ / TS; ++s) {
             __syncthreads();
             for (int jj = tid; jj < TS; jj += WG) {
@@ -363,25 +430,14 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
                 const double rx = J.x[j] - Ox, ry = J.y[j] - Oy, rz = J.z[j] - Oz;
                 const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
                 s_hi[jj] = make_float4(hx, hy, hz, (float)J.m[j]);
-                float hj2 = 0.0f;
-                if (FUSED) {
-                    const double hj = J.Hs[j];
-                    hj2 = (float)(hj * hj);
-                }
                 s_lo[jj] = make_float4((float)(rx - (double)hx), (float)(ry - (double)hy),
-                                       (float)(rz - (double)hz), hj2);
+                                       (float)(rz - (double)hz), 0.0f);
             }
             __syncthreads();
             if (!active) continue;
             for (int h = 0; h < TS / TC; ++h) {
                 const int c = s * (TS / TC) + h;
-                if ((mask >> c) & 1u) {
-                    if (FUSED)
-                        near_chunk_fp32<IPL, NK>(s_hi + h * TC, s_lo + h * TC, I, J,
-                                                 (int64_t)t * TJ + c * TC, Ox, Oy, Oz, xh, yh, zh,
-                                                 xl, yl, zl, hi2f, ii, ax, ay, az, ap);
-                    continue;
-                }
+                if ((mask >> c) & 1u) continue;
                 if (wide)
                     far_chunk_fp32<IPL, true>(s_hi + h * TC, s_lo + h * TC, xh, yh, zh, xl, yl,
                                               zl, ax, ay, az, ap);
@@ -389,6 +445,21 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
                     far_chunk_fp32<IPL, false>(s_hi + h * TC, s_lo + h * TC, xh, yh, zh, xl, yl,
                                                zl, ax, ay, az, ap);
             }
+        }
+    }
+    // the wave's deferred near chunks
+    if (FUSED && nlist > 0) {
+        double xr[IPL], yr[IPL], zr[IPL], Hi[IPL], invHi[IPL], hi2[IPL];
+        int idi[IPL];
+        near_i_state<IPL>(I, ii, Ox, Oy, Oz, xr, yr, zr, Hi, invHi, hi2, idi);
+        wave_lds_sync();
+        for (int q = 0; q < nlist; ++q) {
+            const uint32_t e = s_near[wv].list[q];
+            const int t = b.t0 + (int)(e >> 4);
+            for (int c = 0; c < CHUNKS; ++c)
+                if ((e >> c) & 1u)
+                    near_chunk_slice<IPL, NK>(s_near[wv], J, t, c, Ox, Oy, Oz, xr, yr, zr, Hi,
+                                              invHi, hi2, idi, ax, ay, az, ap);
         }
     }
 
@@ -421,6 +492,7 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
     const double Ox = I.x[io], Oy = I.y[io], Oz = I.z[io];
 
     double xr[IPL], yr[IPL], zr[IPL];
+    float xf[IPL], yf[IPL], zf[IPL], hf[IPL];
     int64_t ii[IPL];
     bool valid[IPL];
     float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -433,12 +505,14 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
         xr[k] = I.x[ii[k]] - Ox;
         yr[k] = I.y[ii[k]] - Oy;
         zr[k] = I.z[ii[k]] - Oz;
-        // the same FP32 box as the FP32 kernel, so both classify tiles alike
-        bl[0] = fminf(bl[0], (float)xr[k]); bl[1] = fminf(bl[1], (float)yr[k]);
-        bl[2] = fminf(bl[2], (float)zr[k]);
-        bh[0] = fmaxf(bh[0], (float)xr[k]); bh[1] = fmaxf(bh[1], (float)yr[k]);
-        bh[2] = fmaxf(bh[2], (float)zr[k]);
-        hmax = fmaxf(hmax, (float)fabs(I.Hs[ii[k]]));
+        // the same FP32 positions as the FP32 kernel, so both classify tiles alike
+        xf[k] = (float)xr[k];
+        yf[k] = (float)yr[k];
+        zf[k] = (float)zr[k];
+        hf[k] = (float)fabs(I.Hs[ii[k]]);
+        bl[0] = fminf(bl[0], xf[k]); bl[1] = fminf(bl[1], yf[k]); bl[2] = fminf(bl[2], zf[k]);
+        bh[0] = fmaxf(bh[0], xf[k]); bh[1] = fmaxf(bh[1], yf[k]); bh[2] = fmaxf(bh[2], zf[k]);
+        hmax = fmaxf(hmax, hf[k]);
     }
     for (int c = 0; c < 3; ++c) {
         bl[c] = wave_min(bl[c]);
@@ -456,8 +530,8 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
     for (int t = b.t0; t < b.t1; ++t) {
         const TileMeta &tm = tiles[t];
         const unsigned mask =
-            active ? near_mask(chunks, t, tm, box_dist2(tm, Ox, Oy, Oz, bl, bh), Ox, Oy, Oz, bl,
-                               bh, hmax)
+            active ? near_mask<IPL>(chunks, t, tm, box_dist2(tm, Ox, Oy, Oz, bl, bh), Ox, Oy, Oz,
+                                    bl, bh, hmax, xf, yf, zf, hf)
                    : 0u;
         if (lane == 0) flags[t - b.t0] = (unsigned char)mask;
         nearv += __builtin_popcount(mask);
@@ -725,8 +799,9 @@ void k_far_fp32c(PairArgs a)
 {
     __shared__ float4 s_hi[TS];
     __shared__ float4 s_lo[TS];
+    __shared__ NearSlice s_near[WG / 64];
     far_block_fp32c<WG, IPL, NK, true>(a.i, a.j, a.tiles, a.chunks, single_block<WG, IPL>(a),
-                                       a.stats, s_hi, s_lo);
+                                       a.stats, s_hi, s_lo, s_near);
 }
 
 // Far chunks only, near chunks flagged (the pass with the grad-h term).
@@ -736,7 +811,7 @@ __global__ __launch_bounds__(WG) void k_far_fp32c_split(PairArgs a)
     __shared__ float4 s_hi[TS];
     __shared__ float4 s_lo[TS];
     far_block_fp32c<WG, 1, 0, false>(a.i, a.j, a.tiles, a.chunks, single_block<WG, 1>(a),
-                                     a.stats, s_hi, s_lo);
+                                     a.stats, s_hi, s_lo, nullptr);
 }
 
 template <int WG, int IPL>
@@ -893,6 +968,7 @@ void k_far_batch(BatchArgs a)
     __shared__ double4 s_buf[TS];
     float4 *s_hi = reinterpret_cast<float4 *>(s_buf);
     float4 *s_lo = s_hi + TS;
+    __shared__ NearSlice s_near[FP64 ? 1 : WG / 64];
     __shared__ int s_item;
     for (;;) {
         __syncthreads();
@@ -905,7 +981,7 @@ void k_far_batch(BatchArgs a)
             far_block_fp64<WG, IPL>(a.j, a.j, a.tiles, a.chunks, b, a.stats, s_buf);
         else
             far_block_fp32c<WG, IPL, NK, true>(a.j, a.j, a.tiles, a.chunks, b, a.stats, s_hi,
-                                               s_lo);
+                                               s_lo, s_near);
     }
 }
 

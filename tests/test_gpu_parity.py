@@ -294,8 +294,15 @@ def test_grad_h_psi(grav, plummer16k, nk, prec):
     n = d["x"].size
     ax, ay, az, pot, psi = grav.compute_psi(*_args(d), nkernel=nk, precision=prec)
     plain = grav.compute(*_args(d), nkernel=nk, precision=prec)
-    for a, b in zip((ax, ay, az, pot), plain):
-        np.testing.assert_allclose(a, b, rtol=1e-12, atol=0)
+    if prec == SG_PREC_FP64:
+        # the same FP64 far kernel and the same FP64 near pair in both passes
+        for a, b in zip((ax, ay, az, pot), plain):
+            np.testing.assert_allclose(a, b, rtol=1e-12, atol=0)
+    else:
+        # the plain FP32C pass evaluates near chunks inside its far kernel, the
+        # psi pass in the near kernel: the same pairs, summed in another order
+        ea, ep = rel_errors(np.stack([ax, ay, az], -1), pot, np.stack(plain[:3], -1), plain[3])
+        assert ea.max() <= 1e-6 and ep.max() <= 1e-7, (ea.max(), ep.max())
     rows = np.unique(np.concatenate([np.arange(0, n, 61), [10, 11, n - 1]]))
     ref = oracle.psi(*_args(d), nkernel=nk, rows=rows)
     tab = oracle.psi(*_args(d), nkernel=nk, rows=rows, tabulated=True)
