@@ -193,12 +193,17 @@ int ensure_capacity(sg_context *c, int64_t n, int64_t padded, int64_t ni)
 
 int auto_jsplit(int niblocks, int wg, int ntiles)
 {
-    // aim for >= 4096 resident waves (256 CUs x 4 SIMDs x 4) when i alone
-    // cannot provide them; never more partitions than tiles
+    // Split the j range until the launch holds about 2^18 waves (some 36
+    // rounds of the 7168 that fit at once at 7 per SIMD): the last round of
+    // a launch with few waves leaves most SIMDs idle, and small blocks also
+    // spread the uneven near-field work.  Partitions keep >= 128 tiles each,
+    // except that small sets may still go down to 2 tiles per partition to
+    // reach 4096 waves.
     const int64_t waves = (int64_t)niblocks * (wg / 64);
-    if (waves >= 4096) return 1;
-    const int64_t want = (4096 + waves - 1) / waves;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(want, std::max(1, ntiles / 2)));
+    const int64_t want = ((int64_t)1 << 18) / std::max<int64_t>(1, waves);
+    const int64_t small = std::min<int64_t>(ntiles / 2, (4096 + waves - 1) / waves);
+    const int64_t cap = std::max<int64_t>(ntiles / 128, small);
+    return (int)std::max<int64_t>(1, std::min(want, cap));
 }
 
 bool env_flag(const char *name)
