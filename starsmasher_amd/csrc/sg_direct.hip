@@ -71,22 +71,16 @@ struct Group {
 };
 
 // ------------------------------------------------ far block, FP32 + comp.
-// One Newtonian pair (i <- j) in FP32.  FIRST starts the partial sums (the
-// first j of a fold writes them, which saves clearing them).
-template <bool WIDE, bool FIRST>
-__device__ __forceinline__ void far_pair_fp32(const float4 &p, const float4 &q, float xh, float yh,
-                                              float zh, float xl, float yl, float zl, float &fx,
+// One Newtonian pair (i <- j) in FP32 from its difference dr and m_j.  FIRST
+// starts the partial sums (the first j of a fold writes them, which saves
+// clearing them).
+template <bool FIRST>
+__device__ __forceinline__ void far_pair_fp32(float dx, float dy, float dz, float mj, float &fx,
                                               float &fy, float &fz, float &fp)
 {
-    float dx = p.x - xh, dy = p.y - yh, dz = p.z - zh;
-    if (WIDE) {
-        dx += q.x - xl;
-        dy += q.y - yl;
-        dz += q.z - zl;
-    }
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float ri = __builtin_amdgcn_rsqf(r2);
-    const float mr = p.w * ri;
+    const float mr = mj * ri;
     const float mr3 = mr * ri * ri;
     if (FIRST) {
         fx = mr3 * dx;
@@ -101,37 +95,68 @@ __device__ __forceinline__ void far_pair_fp32(const float4 &p, const float4 &q, 
     }
 }
 
+// m_j travels in the FP64 stage as the low half of the w slot.
+__device__ __forceinline__ double pack_mass(float m)
+{
+    const float2 v = make_float2(m, 0.0f);
+    return __builtin_bit_cast(double, v);
+}
+
+// The difference of one pair for the FP32 body.  Far tiles use hi-only
+// differences (x_j,hi - x_i,hi).  WIDE tiles (close relative to the wave's
+// spread about the origin) form x_j - x_i in FP64 from the FP64 stage and
+// round it once (two ops per axis): exact to half an ulp of r, with no
+// shared error from the rounding of x_i.
+template <bool WIDE>
+__device__ __forceinline__ void far_diff(const float4 *ph, const double4 *pd, int jj, float xh,
+                                         float yh, float zh, double xr, double yr, double zr,
+                                         float &dx, float &dy, float &dz, float &mj)
+{
+    if (WIDE) {
+        const double4 q = pd[jj];
+        dx = (float)(q.x - xr);
+        dy = (float)(q.y - yr);
+        dz = (float)(q.z - zr);
+        mj = __builtin_bit_cast(float2, q.w).x;
+    } else {
+        const float4 p = ph[jj];
+        dx = p.x - xh;
+        dy = p.y - yh;
+        dz = p.z - zh;
+        mj = p.w;
+    }
+}
+
 // One chunk (TC j) of far pairs from LDS.  The FOLD j of a fold are
 // unrolled, so the LDS reads use immediate offsets from one address per fold.
 template <int IPL, bool WIDE>
-__device__ __forceinline__ void far_chunk_fp32(const float4 *s_hi, const float4 *s_lo,
+__device__ __forceinline__ void far_chunk_fp32(const float4 *s_hi, const double4 *s_p,
                                                const float (&xh)[IPL], const float (&yh)[IPL],
-                                               const float (&zh)[IPL], const float (&xl)[IPL],
-                                               const float (&yl)[IPL], const float (&zl)[IPL],
+                                               const float (&zh)[IPL], const double (&xr)[IPL],
+                                               const double (&yr)[IPL], const double (&zr)[IPL],
                                                double (&ax)[IPL], double (&ay)[IPL],
                                                double (&az)[IPL], double (&ap)[IPL])
 {
 #pragma unroll 1
     for (int j0 = 0; j0 < TC; j0 += FOLD) {
         const float4 *ph = s_hi + j0;
-        const float4 *pl = s_lo + j0;
+        const double4 *pd = s_p + j0;
         float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
-        {
-            const float4 p = ph[0];
-            const float4 q = WIDE ? pl[0] : p;
 #pragma unroll
-            for (int k = 0; k < IPL; ++k)
-                far_pair_fp32<WIDE, true>(p, q, xh[k], yh[k], zh[k], xl[k], yl[k], zl[k], fx[k],
-                                          fy[k], fz[k], fp[k]);
+        for (int k = 0; k < IPL; ++k) {
+            float dx, dy, dz, mj;
+            far_diff<WIDE>(ph, pd, 0, xh[k], yh[k], zh[k], xr[k], yr[k], zr[k], dx, dy, dz, mj);
+            far_pair_fp32<true>(dx, dy, dz, mj, fx[k], fy[k], fz[k], fp[k]);
         }
 #pragma unroll
         for (int jj = 1; jj < FOLD; ++jj) {
-            const float4 p = ph[jj];
-            const float4 q = WIDE ? pl[jj] : p;
 #pragma unroll
-            for (int k = 0; k < IPL; ++k)
-                far_pair_fp32<WIDE, false>(p, q, xh[k], yh[k], zh[k], xl[k], yl[k], zl[k], fx[k],
-                                           fy[k], fz[k], fp[k]);
+            for (int k = 0; k < IPL; ++k) {
+                float dx, dy, dz, mj;
+                far_diff<WIDE>(ph, pd, jj, xh[k], yh[k], zh[k], xr[k], yr[k], zr[k], dx, dy, dz,
+                               mj);
+                far_pair_fp32<false>(dx, dy, dz, mj, fx[k], fy[k], fz[k], fp[k]);
+            }
         }
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
@@ -208,19 +233,14 @@ __device__ __forceinline__ void near_pair(double dx, double dy, double dz, doubl
     }
 }
 
-// FP64 i-side state of the near pairs (positions relative to the origin).
+// FP64 i-side softening state of the near pairs.
 template <int IPL>
 __device__ __forceinline__ void near_i_state(const SortedSet &I, const int64_t (&ii)[IPL],
-                                             double Ox, double Oy, double Oz, double (&xr)[IPL],
-                                             double (&yr)[IPL], double (&zr)[IPL],
                                              double (&Hi)[IPL], double (&invHi)[IPL],
                                              double (&hi2)[IPL], int (&idi)[IPL])
 {
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
-        xr[k] = I.x[ii[k]] - Ox;
-        yr[k] = I.y[ii[k]] - Oy;
-        zr[k] = I.z[ii[k]] - Oz;
         Hi[k] = fabs(I.Hs[ii[k]]);
         invHi[k] = 1.0 / Hi[k];
         hi2[k] = Hi[k] * Hi[k];
@@ -341,7 +361,7 @@ template <int WG, int IPL, int NK, bool FUSED>
 __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const SortedSet &J,
                                                 const TileMeta *tiles, const TileMeta *chunks,
                                                 const Block &b, unsigned long long *stats,
-                                                float4 *s_hi, float4 *s_lo, NearSlice *s_near)
+                                                float4 *s_hi, double4 *s_p, NearSlice *s_near)
 {
     constexpr int WI = 64 * IPL;   // i per wave
     constexpr int GI = WG * IPL;   // i per workgroup
@@ -351,7 +371,8 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
     const int64_t io = min(b.ibase + GI / 2, b.iend - 1);
     const double Ox = I.x[io], Oy = I.y[io], Oz = I.z[io];
 
-    float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL], hf[IPL];
+    float xh[IPL], yh[IPL], zh[IPL], hf[IPL];
+    double xr[IPL], yr[IPL], zr[IPL];
     int64_t ii[IPL];
     bool valid[IPL];
     float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -361,13 +382,12 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         const int64_t i = b.ibase + wv * WI + k * 64 + lane;
         valid[k] = i < b.iend;
         ii[k] = valid[k] ? i : b.iend - 1;
-        const double rx = I.x[ii[k]] - Ox, ry = I.y[ii[k]] - Oy, rz = I.z[ii[k]] - Oz;
-        xh[k] = (float)rx;
-        yh[k] = (float)ry;
-        zh[k] = (float)rz;
-        xl[k] = (float)(rx - (double)xh[k]);
-        yl[k] = (float)(ry - (double)yh[k]);
-        zl[k] = (float)(rz - (double)zh[k]);
+        xr[k] = I.x[ii[k]] - Ox;
+        yr[k] = I.y[ii[k]] - Oy;
+        zr[k] = I.z[ii[k]] - Oz;
+        xh[k] = (float)xr[k];
+        yh[k] = (float)yr[k];
+        zh[k] = (float)zr[k];
         hf[k] = (float)fabs(I.Hs[ii[k]]);
         bl[0] = fminf(bl[0], xh[k]); bl[1] = fminf(bl[1], yh[k]); bl[2] = fminf(bl[2], zh[k]);
         bh[0] = fmaxf(bh[0], xh[k]); bh[1] = fmaxf(bh[1], yh[k]); bh[2] = fmaxf(bh[2], zh[k]);
@@ -414,9 +434,9 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
             }
         }
         if (FUSED && near_tile_now) {
-            double xr[IPL], yr[IPL], zr[IPL], Hi[IPL], invHi[IPL], hi2[IPL];
+            double Hi[IPL], invHi[IPL], hi2[IPL];
             int idi[IPL];
-            near_i_state<IPL>(I, ii, Ox, Oy, Oz, xr, yr, zr, Hi, invHi, hi2, idi);
+            near_i_state<IPL>(I, ii, Hi, invHi, hi2, idi);
             for (int c = 0; c < CHUNKS; ++c)
                 if ((mask >> c) & 1u)
                     near_chunk_slice<IPL, NK>(s_near[wv], J, t, c, Ox, Oy, Oz, xr, yr, zr, Hi,
@@ -428,10 +448,9 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
             for (int jj = tid; jj < TS; jj += WG) {
                 const int64_t j = (int64_t)t * TJ + s * TS + jj;
                 const double rx = J.x[j] - Ox, ry = J.y[j] - Oy, rz = J.z[j] - Oz;
-                const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
-                s_hi[jj] = make_float4(hx, hy, hz, (float)J.m[j]);
-                s_lo[jj] = make_float4((float)(rx - (double)hx), (float)(ry - (double)hy),
-                                       (float)(rz - (double)hz), 0.0f);
+                const float mf = (float)J.m[j];
+                s_hi[jj] = make_float4((float)rx, (float)ry, (float)rz, mf);
+                s_p[jj] = make_double4(rx, ry, rz, pack_mass(mf));
             }
             __syncthreads();
             if (!active) continue;
@@ -439,19 +458,19 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
                 const int c = s * (TS / TC) + h;
                 if ((mask >> c) & 1u) continue;
                 if (wide)
-                    far_chunk_fp32<IPL, true>(s_hi + h * TC, s_lo + h * TC, xh, yh, zh, xl, yl,
-                                              zl, ax, ay, az, ap);
+                    far_chunk_fp32<IPL, true>(s_hi + h * TC, s_p + h * TC, xh, yh, zh, xr, yr, zr,
+                                              ax, ay, az, ap);
                 else
-                    far_chunk_fp32<IPL, false>(s_hi + h * TC, s_lo + h * TC, xh, yh, zh, xl, yl,
-                                               zl, ax, ay, az, ap);
+                    far_chunk_fp32<IPL, false>(s_hi + h * TC, s_p + h * TC, xh, yh, zh, xr, yr, zr,
+                                               ax, ay, az, ap);
             }
         }
     }
     // the wave's deferred near chunks
     if (FUSED && nlist > 0) {
-        double xr[IPL], yr[IPL], zr[IPL], Hi[IPL], invHi[IPL], hi2[IPL];
+        double Hi[IPL], invHi[IPL], hi2[IPL];
         int idi[IPL];
-        near_i_state<IPL>(I, ii, Ox, Oy, Oz, xr, yr, zr, Hi, invHi, hi2, idi);
+        near_i_state<IPL>(I, ii, Hi, invHi, hi2, idi);
         wave_lds_sync();
         for (int q = 0; q < nlist; ++q) {
             const uint32_t e = s_near[wv].list[q];
@@ -798,10 +817,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(IPL == 1 ? 7
 void k_far_fp32c(PairArgs a)
 {
     __shared__ float4 s_hi[TS];
-    __shared__ float4 s_lo[TS];
+    __shared__ double4 s_p[TS];
     __shared__ NearSlice s_near[WG / 64];
     far_block_fp32c<WG, IPL, NK, true>(a.i, a.j, a.tiles, a.chunks, single_block<WG, IPL>(a),
-                                       a.stats, s_hi, s_lo, s_near);
+                                       a.stats, s_hi, s_p, s_near);
 }
 
 // Far chunks only, near chunks flagged (the pass with the grad-h term).
@@ -809,9 +828,9 @@ template <int WG>
 __global__ __launch_bounds__(WG) void k_far_fp32c_split(PairArgs a)
 {
     __shared__ float4 s_hi[TS];
-    __shared__ float4 s_lo[TS];
+    __shared__ double4 s_p[TS];
     far_block_fp32c<WG, 1, 0, false>(a.i, a.j, a.tiles, a.chunks, single_block<WG, 1>(a),
-                                     a.stats, s_hi, s_lo, nullptr);
+                                     a.stats, s_hi, s_p, nullptr);
 }
 
 template <int WG, int IPL>
@@ -964,10 +983,10 @@ template <int WG, int IPL, bool FP64, int NK>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(IPL == 1 && !FP64 ? 7 : 1)))
 void k_far_batch(BatchArgs a)
 {
-    // one TS-entry FP64 tile, or the two TS-entry FP32 tiles (hi, lo)
-    __shared__ double4 s_buf[TS];
+    // one TS-entry FP64 tile, or the FP32 hi stage followed by the FP64 stage
+    __shared__ double4 s_buf[TS + TS / 2];
     float4 *s_hi = reinterpret_cast<float4 *>(s_buf);
-    float4 *s_lo = s_hi + TS;
+    double4 *s_p = s_buf + TS / 2;
     __shared__ NearSlice s_near[FP64 ? 1 : WG / 64];
     __shared__ int s_item;
     for (;;) {
@@ -981,7 +1000,7 @@ void k_far_batch(BatchArgs a)
             far_block_fp64<WG, IPL>(a.j, a.j, a.tiles, a.chunks, b, a.stats, s_buf);
         else
             far_block_fp32c<WG, IPL, NK, true>(a.j, a.j, a.tiles, a.chunks, b, a.stats, s_hi,
-                                               s_lo, s_near);
+                                               s_p, s_near);
     }
 }
 
