@@ -219,6 +219,32 @@ def test_far_path_dominates_with_small_h(grav, prec):
 
 
 @pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+@pytest.mark.parametrize("wg,ipl", [(64, 1), (256, 1), (128, 2)])
+def test_waves_near_every_tile(grav, prec, wg, ipl):
+    """40 gas particles with H = 100 (ten times the set's radius): every pair
+    they take part in is softened, so their waves find every one of the 256
+    tiles near (more than the 128 tiles a wave's deferred near list holds,
+    which sends the rest through the list-overflow path), and their chunks
+    are near for every wave."""
+    d = ics.plummer(65536, seed=21)
+    rng = np.random.default_rng(5)
+    big = rng.choice(65536, 40, replace=False)
+    d["range2"][big] = 1.0e4
+    old = grav.get_config(prec)
+    grav.set_config(prec, wg, ipl, 0)
+    try:
+        res = grav.compute(*_args(d), nkernel=0, precision=prec)
+        visits, nearv = grav.last_tile_stats()
+    finally:
+        grav.set_config(prec, *old)
+    print(f"[bigH {wg}x{ipl}] near chunk visits {nearv}/{visits}")
+    assert nearv > 0
+    rows = np.unique(np.concatenate([big, rng.choice(65536, 400, replace=False)]))
+    ref = oracle.direct(*_args(d), nkernel=0, rows=rows)
+    _check(f"bigH {wg}x{ipl}", prec, [r[rows] for r in res], ref)
+
+
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
 def test_giant_with_point_mass_rows(grav, prec):
     """Masses over 4 decades, H over 3 decades and a 30% point mass (no self
     potential, softening H = 0.2), against the oracle on sampled rows."""
