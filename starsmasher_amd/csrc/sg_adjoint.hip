@@ -174,81 +174,6 @@ struct Lane32 {
     float xh, yh, zh, xl, yl, zl, m, wx, wy, wz, u, h2;   // h2 = H_k^2
 };
 
-// One near chunk (TC j) of the FP32C adjoint inside the far kernel: the FP32
-// pair body with the lanes whose pair may be softened (r^2 < max(H_k^2,
-// H_l^2) in FP32, r = 0 included) masked out, then adj_pair64 in FP64 for
-// those lanes of the j a wave ballot marked.
-template <int NK>
-__device__ __forceinline__ void adj_near_chunk_fp32(const float4 *s_p, const float4 *s_w,
-                                                    const float4 *s_lo, const Lane32 &q,
-                                                    const AdjArgs &a, int64_t jbase, int64_t i,
-                                                    double Ox, double Oy, double Oz,
-                                                    double (&acc)[9])
-{
-    uint64_t jmask = 0;
-    for (int j0 = 0; j0 < TC; j0 += AFOLD) {
-        float g1x = 0.0f, g1y = 0.0f, g1z = 0.0f, g2 = 0.0f;
-        float g3x = 0.0f, g3y = 0.0f, g3z = 0.0f, gm = 0.0f;
-        for (int jj = j0; jj < j0 + AFOLD; ++jj) {
-            const float4 p = s_p[jj];
-            const float4 w = s_w[jj];
-            const float4 l = s_lo[jj];   // l.w = H_l^2
-            const float dx = (p.x - q.xh) + (l.x - q.xl);
-            const float dy = (p.y - q.yh) + (l.y - q.yl);
-            const float dz = (p.z - q.zh) + (l.z - q.zl);
-            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            const bool soft = r2 < fmaxf(q.h2, l.w);
-            const float ri = soft ? 0.0f : __builtin_amdgcn_rsqf(r2);
-            const float ri2 = ri * ri;
-            const float ri3 = ri2 * ri;
-            const float wkdr = fmaf(q.wx, dx, fmaf(q.wy, dy, q.wz * dz));
-            const float wldr = fmaf(w.x, dx, fmaf(w.y, dy, w.z * dz));
-            const float t = fmaf(p.w, wkdr, -q.m * wldr);
-            const float sc = fmaf(p.w, q.u, q.m * w.w);
-            const float c = ri3 * fmaf(3.0f * t, ri2, -sc);
-            g1x = fmaf(ri3, w.x, g1x);
-            g1y = fmaf(ri3, w.y, g1y);
-            g1z = fmaf(ri3, w.z, g1z);
-            g2 = fmaf(ri3, p.w, g2);
-            g3x = fmaf(c, dx, g3x);
-            g3y = fmaf(c, dy, g3y);
-            g3z = fmaf(c, dz, g3z);
-            gm = fmaf(w.w, ri, fmaf(ri3, wldr, gm));
-            if (__ballot(soft)) jmask |= 1ull << jj;
-        }
-        acc[0] += (double)g1x;
-        acc[1] += (double)g1y;
-        acc[2] += (double)g1z;
-        acc[3] += (double)g2;
-        acc[4] += (double)g3x;
-        acc[5] += (double)g3y;
-        acc[6] += (double)g3z;
-        acc[7] += (double)gm;
-    }
-    const SortedSet &P = a.p;
-    while (jmask) {
-        const int jj = __builtin_ctzll(jmask);
-        jmask &= jmask - 1;
-        const float4 p = s_p[jj];
-        const float4 l = s_lo[jj];
-        const float dx = (p.x - q.xh) + (l.x - q.xl);
-        const float dy = (p.y - q.yh) + (l.y - q.yl);
-        const float dz = (p.z - q.zh) + (l.z - q.zl);
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        if (!(r2 < fmaxf(q.h2, l.w))) continue;   // this lane took the FP32 pair
-        const int64_t j = jbase + jj;
-        const double Hs = P.Hs[i], Hk = fabs(Hs), hl = fabs(P.Hs[j]);
-        Acc9 g = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        adj_pair64<true, NK>((P.x[j] - Ox) - (P.x[i] - Ox), (P.y[j] - Oy) - (P.y[i] - Oy),
-                             (P.z[j] - Oz) - (P.z[i] - Oz), Hk, 1.0 / Hk, P.m[i], a.wx[i],
-                             a.wy[i], a.wz[i], a.u[i], Hs > 0.0, hl, P.m[j], a.wx[j], a.wy[j],
-                             a.wz[j], a.u[j], P.idx[j] == P.idx[i], g);
-        for (int k = 0; k < 7; ++k) acc[k] += g[k];
-        acc[7] -= g[7];
-        acc[8] += g[8];
-    }
-}
-
 // One chunk (TC j) of far pairs in FP32.  acc: G1 (3), G2, G3 (3), GM' with
 // GM' = sum (u_l / r + w_l.dr / r^3) = -GM, and GH.
 template <bool WIDE>
@@ -336,9 +261,10 @@ __device__ __forceinline__ void adj_tile_range(const AdjArgs &a, int jp, int &t0
     t1 = (int)((int64_t)a.ntiles * (jp + 1) / a.jsplit);
 }
 
-// Grid: niblocks x jsplit workgroups of WG lanes, one i per lane.  FP32C
-// (FP64 = false) takes the near chunks itself; FP64 flags them (chunk
-// bits) for k_adj_near.
+// Grid: niblocks x jsplit workgroups of WG lanes, one i per lane.  Near
+// chunks are flagged (chunk bits) for k_adj_near in both precisions: kept
+// out of the far kernel, they leave the FP32C far loop a register budget of
+// 4 waves per SIMD without spills in it.
 template <int WG, bool FP64, int NK>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(FP64 ? 1 : 4)))
 void k_adj_far(AdjArgs a)
@@ -405,9 +331,9 @@ void k_adj_far(AdjArgs a)
             active ? near_mask<1>(a.chunks, t, tm, d2, Ox, Oy, Oz, bl, bh, hmax, px, py, pz, ph)
                    : 0u;
         const bool wide = uniform_i(d2 < wide_d2) != 0;
-        if (FP64 && lane == 0) flags[t] = (unsigned char)mask;
+        if (lane == 0) flags[t] = (unsigned char)mask;
         nearv += __builtin_popcount(mask);
-        if (FP64 && WG == 64 && mask == (1u << CHUNKS) - 1) continue;
+        if (WG == 64 && mask == (1u << CHUNKS) - 1) continue;
         for (int st = 0; st < TJ / ATS; ++st) {
             __syncthreads();
             for (int jj = tid; jj < ATS; jj += WG) {
@@ -431,12 +357,7 @@ void k_adj_far(AdjArgs a)
             for (int h = 0; h < ATS / TC; ++h) {
                 const int c = st * (ATS / TC) + h;
                 const int o = h * TC;
-                if ((mask >> c) & 1u) {
-                    if (!FP64)
-                        adj_near_chunk_fp32<NK>(s_p32 + o, s_w32 + o, s_lo32 + o, q, a,
-                                                (int64_t)t * TJ + c * TC, i, Ox, Oy, Oz, acc);
-                    continue;
-                }
+                if ((mask >> c) & 1u) continue;   // k_adj_near
                 if (FP64)
                     adj_chunk_fp64(s_p64 + o, s_w64 + o, rx, ry, rz, mk, wkx, wky, wkz, uk, acc);
                 else if (wide)
@@ -593,8 +514,18 @@ hipError_t launch_adj_far(int precision, const AdjArgs &a, int wg, hipStream_t s
 hipError_t launch_adj_near(int precision, const AdjArgs &a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
-    if (precision == 0) return hipSuccess;   // the FP32C far kernel took the near chunks
+    // FP64 near pairs for both precisions; the FP32C pass seeds the FP64
+    // reciprocal square root in FP32 and refines it with one Newton step
     const dim3 grid((unsigned)(a.ngroups * a.jsplit)), block(64 * ANEAR_WAVES);
+    if (precision == 0) {
+        switch (a.nkernel) {
+        case 0: hipLaunchKernelGGL((k_adj_near<true, 0>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_adj_near<true, 1>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_adj_near<true, 2>), grid, block, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (a.nkernel) {
     case 0: hipLaunchKernelGGL((k_adj_near<false, 0>), grid, block, 0, s, a); break;
     case 1: hipLaunchKernelGGL((k_adj_near<false, 1>), grid, block, 0, s, a); break;
