@@ -13,10 +13,11 @@ constexpr float FAR_MARGIN = 1.0e-3f;   // relative guard on the box-distance te
 // Hi-only differences x_j,hi - x_i,hi carry a relative error of about
 // 2^-24 (|x_i - O| + |x_j - O| + r) / r <= 2^-24 (2 + 2 |x_i - O| / r), with
 // |x_i - O| <= sqrt(3) reach (reach = per-axis max over the wave).  Tiles
-// nearer than sqrt(3) WIDE_K reach use hi+lo differences, which bounds the
-// hi-only ones by 2^-24 (2 + 2 / WIDE_K) = 6 ulp (3.6e-7).  Waves that
-// straddle a jump of the space-filling curve have a large reach; with
-// WIDE_K = 0.5 about 2% of tiles take the hi+lo body on the giant-star set.
+// nearer than sqrt(3) WIDE_K reach form the difference in FP64 and round it
+// once, which bounds the hi-only ones by 2^-24 (2 + 2 / WIDE_K) = 6 ulp
+// (3.6e-7).  Waves that straddle a jump of the space-filling curve have a
+// large reach; with WIDE_K = 0.5 about 2% of tiles take the FP64-difference
+// body on the giant-star set.
 constexpr float WIDE_K = 0.5f;
 
 __device__ __forceinline__ float uniform_f(float v)

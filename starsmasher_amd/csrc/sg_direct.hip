@@ -5,31 +5,31 @@
 // 256-wide shared-memory tiles, every pair through the full three-way softened
 // branch, float-float (Fast2Sum) accumulation of every term.
 //
-// Design here (wave64, VALU-bound, no MFMA -- a pairwise sum is not a
-// contraction):
+// Design here (wave64, VALU-bound):
 //   * particles arrive sorted along a Morton curve (sg_prep.hip), so an
 //     i-group of 64*IPL consecutive particles (one wave) and a j-tile of TJ
-//     consecutive particles are both spatially compact;
-//   * FAR kernel: each wave holds IPL i-particles per lane in registers and
-//     tests every j-tile once against its own bounding box.  If the boxes are
-//     farther apart than the largest support radius of both, every pair in
-//     the tile is Newtonian (the grav_force_direct.cu:69-75 branch) and runs a
-//     branch-free 14-instruction body per pair (3 sub, 3 for r^2, rsq, 3 mul,
-//     3 fma, 1 add).  Otherwise the tile is flagged for the near kernel;
-//   * j data for far tiles is staged through LDS as FP32 {x_hi, y_hi, z_hi, m}
-//     and {x_lo, y_lo, z_lo} relative to an FP64 origin taken per workgroup,
+//     consecutive particles (four chunks of TC) are both spatially compact;
+//   * each wave holds IPL i-particles per lane in registers and classifies
+//     every (wave, tile) and then every (wave, chunk) with box tests and a
+//     per-lane test (near_mask): a far chunk has every pair Newtonian (the
+//     grav_force_direct.cu:69-75 branch) and runs a branch-free body of 13
+//     VALU operations plus one v_rsq_f32 per pair;
+//   * j data is staged through LDS per workgroup as FP32 {x_hi, y_hi, z_hi, m}
+//     and as FP64 {x, y, z} relative to an FP64 origin taken per workgroup,
 //     written from coalesced FP64 global loads and read back as same-address
-//     (broadcast) ds_read_b128.  Tiles close to the wave use hi+lo
-//     differences; distant ones hi only;
+//     (broadcast) LDS reads.  Distant tiles use hi-only differences; tiles
+//     close to the wave form the difference in FP64 and round it once;
 //   * FP32 partial sums over FOLD j are folded into FP64 accumulators (the
-//     cross-tile reduction is FP64);
-//   * NEAR kernel: the flagged (wave, tile) pairs, every pair in FP64 (the
-//     Wendland polynomials carry coefficients up to 9009 that cancel to O(1)
-//     near q = 1, which FP32 cannot resolve);
+//     cross-tile and cross-partition reductions are FP64);
+//   * near chunks (pairs that may be softened) are listed per wave and run
+//     after its far loop in the wave's own LDS slice: FP64-formed
+//     differences, the FP32 body for the Newtonian lanes, FP64 near_pair for
+//     the softened ones (the Wendland polynomials carry coefficients up to
+//     9009 that cancel to O(1) near q = 1, which FP32 cannot resolve);
+//   * the FP64 comparison pass and the grad-h pass flag near chunks for a
+//     separate near kernel instead;
 //   * REDUCE kernel: fixed-order FP64 sum over j-partitions, scattered back to
 //     input order.
-// The far kernel carries no FP64 softening code, so its register budget (and
-// occupancy) is set by the 14-instruction loop alone.
 //
 // Two launch forms share the block-level code: one particle set (grid over
 // i-blocks x j-partitions), and a batch of independent sets in one persistent
@@ -400,7 +400,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         reach = fmaxf(reach, fmaxf(-bl[c], bh[c]));
     }
     hmax = wave_max(hmax);
-    // close tiles take hi+lo differences (reach is per axis, so the
+    // close tiles take FP64-formed differences (reach is per axis, so the
     // Euclidean reach is at most sqrt(3) times it)
     const float wide_d2 = 3.0f * (WIDE_K * reach) * (WIDE_K * reach);
 
