@@ -191,18 +191,20 @@ int ensure_capacity(sg_context *c, int64_t n, int64_t padded, int64_t ni)
     return SG_OK;
 }
 
-int auto_jsplit(int niblocks, int wg, int ntiles)
+int auto_jsplit(int niblocks, int wg, int ntiles, int64_t ni)
 {
     // Split the j range until the launch holds about 2^18 waves (some 36
     // rounds of the 7168 that fit at once at 7 per SIMD): the last round of
     // a launch with few waves leaves most SIMDs idle, and small blocks also
-    // spread the uneven near-field work.  Partitions keep >= 128 tiles each,
-    // except that small sets may still go down to 2 tiles per partition to
-    // reach 4096 waves.
-    const int64_t waves = (int64_t)niblocks * (wg / 64);
-    const int64_t want = ((int64_t)1 << 18) / std::max<int64_t>(1, waves);
-    const int64_t small = std::min<int64_t>(ntiles / 2, (4096 + waves - 1) / waves);
-    const int64_t cap = std::max<int64_t>(ntiles / 128, small);
+    // spread the uneven near-field work.  Partitions keep >= 4 tiles, there
+    // are at most 64 (more measured slower at 2^17 particles), and the FP64
+    // partial sums (32 B per particle and partition) stay within 2^24
+    // particle-partitions (512 MB).
+    const int64_t waves = std::max<int64_t>(1, (int64_t)niblocks * (wg / 64));
+    const int64_t want = (((int64_t)1 << 18) + waves - 1) / waves;
+    const int64_t slabs = ((int64_t)1 << 24) / std::max<int64_t>(1, ni);
+    const int64_t cap =
+        std::min<int64_t>(64, std::min<int64_t>(std::max(1, ntiles / 4), std::max<int64_t>(1, slabs)));
     return (int)std::max<int64_t>(1, std::min(want, cap));
 }
 
@@ -387,7 +389,7 @@ int compute_device_impl(sg_context *c, int n, int ibeg, int ni, const double *x,
     a.niblocks = (int)((ni + (int64_t)wg * ipl - 1) / ((int64_t)wg * ipl));
     a.ngroups = (int64_t)a.niblocks * (wg / 64);
     a.jsplit = c->jsplit[precision] > 0 ? std::min(c->jsplit[precision], ntiles)
-                                        : auto_jsplit(a.niblocks, wg, ntiles);
+                                        : auto_jsplit(a.niblocks, wg, ntiles, ni);
     a.ax = ax;
     a.ay = ay;
     a.az = az;
@@ -856,7 +858,7 @@ int sg_vjp_device(sg_context *c, int n, const double *x, const double *y, const 
     const int wg = sg::adj_config_supported(c->wg[precision]) ? c->wg[precision] : 256;
     a.niblocks = (int)((n + (int64_t)wg - 1) / wg);
     a.ngroups = (int64_t)a.niblocks * (wg / 64);
-    a.jsplit = auto_jsplit(a.niblocks, wg, ntiles);
+    a.jsplit = auto_jsplit(a.niblocks, wg, ntiles, n);
     a.gx = gx;
     a.gy = gy;
     a.gz = gz;
