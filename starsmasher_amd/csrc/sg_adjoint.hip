@@ -278,8 +278,8 @@ void k_adj_far(AdjArgs a)
 
     const int ib = (int)(blockIdx.x % (unsigned)a.niblocks);
     const int jp = (int)(blockIdx.x / (unsigned)a.niblocks);
-    int t0, t1;
-    adj_tile_range(a, jp, t0, t1);
+    // partition jp takes every jsplit-th tile (near tiles spread out)
+    const int t0 = jp, t1 = a.ntiles, tstep = a.jsplit;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const SortedSet &P = a.p;
     const int64_t ibase = (int64_t)ib * WG, iend = a.n;
@@ -324,7 +324,7 @@ void k_adj_far(AdjArgs a)
 
     double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     unsigned nearv = 0;
-    for (int t = t0; t < t1; ++t) {
+    for (int t = t0; t < t1; t += tstep) {
         const TileMeta &tm = a.tiles[t];
         const float d2 = box_dist2(tm, Ox, Oy, Oz, bl, bh);
         const unsigned mask =
@@ -377,7 +377,7 @@ void k_adj_far(AdjArgs a)
         part[4 * a.n + i] = acc[8];
     }
     if (a.stats && lane == 0) {
-        atomicAdd(&a.stats[0], (unsigned long long)(t1 - t0) * CHUNKS);
+        atomicAdd(&a.stats[0], (unsigned long long)((t1 - t0 + tstep - 1) / tstep) * CHUNKS);
         atomicAdd(&a.stats[1], (unsigned long long)nearv);
     }
 }

@@ -53,7 +53,8 @@ static_assert(TS % TC == 0 && TC % FOLD == 0, "stage = whole chunks = whole fold
 struct Block {
     int64_t ibase;          // first (sorted) i of the workgroup
     int64_t iend;           // end of the i segment the block lies in
-    int t0, t1;             // j-tile range
+    int t0, t1, tstep;      // j-tiles t0, t0 + tstep, ... below t1
+    int fbase;              // near flags are indexed by t - fbase
     unsigned char *flags;   // near flags of the workgroup's first wave, [t - t0]
     int64_t fstride;        // flag stride between consecutive waves
     double *part;           // partial sums: part[c * pstride + i], c = ax, ay, az, pot
@@ -414,7 +415,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
     unsigned nearv = 0;
     int nlist = 0;   // wave-uniform
     unsigned char *flags = b.flags + wv * b.fstride;
-    for (int t = b.t0; t < b.t1; ++t) {
+    for (int t = b.t0; t < b.t1; t += b.tstep) {
         const TileMeta &tm = tiles[t];
         const float d2 = box_dist2(tm, Ox, Oy, Oz, bl, bh);
         const unsigned mask =
@@ -424,10 +425,11 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         nearv += __builtin_popcount(mask);
         bool near_tile_now = false;   // a full list: this tile's near chunks run below
         if (!FUSED) {
-            if (lane == 0) flags[t - b.t0] = (unsigned char)mask;
+            if (lane == 0) flags[t - b.fbase] = (unsigned char)mask;
         } else if (mask) {
             if (nlist < NLIST) {
-                if (lane == 0) s_near[wv].list[nlist] = ((uint32_t)(t - b.t0) << 4) | mask;
+                if (lane == 0)
+                    s_near[wv].list[nlist] = ((uint32_t)((t - b.t0) / b.tstep) << 4) | mask;
                 ++nlist;
             } else {
                 near_tile_now = true;
@@ -474,7 +476,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         wave_lds_sync();
         for (int q = 0; q < nlist; ++q) {
             const uint32_t e = s_near[wv].list[q];
-            const int t = b.t0 + (int)(e >> 4);
+            const int t = b.t0 + (int)(e >> 4) * b.tstep;
             for (int c = 0; c < CHUNKS; ++c)
                 if ((e >> c) & 1u)
                     near_chunk_slice<IPL, NK>(s_near[wv], J, t, c, Ox, Oy, Oz, xr, yr, zr, Hi,
@@ -491,7 +493,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         b.part[3 * b.pstride + ii[k]] = ap[k];
     }
     if (stats && lane == 0) {
-        atomicAdd(&stats[0], (unsigned long long)(b.t1 - b.t0) * CHUNKS);
+        atomicAdd(&stats[0], (unsigned long long)((b.t1 - b.t0 + b.tstep - 1) / b.tstep) * CHUNKS);
         atomicAdd(&stats[1], (unsigned long long)nearv);
     }
 }
@@ -546,13 +548,13 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
     const bool active = uniform_i(b.ibase + wv * WI < b.iend) != 0;
     unsigned nearv = 0;
     unsigned char *flags = b.flags + wv * b.fstride;
-    for (int t = b.t0; t < b.t1; ++t) {
+    for (int t = b.t0; t < b.t1; t += b.tstep) {
         const TileMeta &tm = tiles[t];
         const unsigned mask =
             active ? near_mask<IPL>(chunks, t, tm, box_dist2(tm, Ox, Oy, Oz, bl, bh), Ox, Oy, Oz,
                                     bl, bh, hmax, xf, yf, zf, hf)
                    : 0u;
-        if (lane == 0) flags[t - b.t0] = (unsigned char)mask;
+        if (lane == 0) flags[t - b.fbase] = (unsigned char)mask;
         nearv += __builtin_popcount(mask);
         if (WG == 64 && mask == (1u << CHUNKS) - 1) continue;
         for (int s = 0; s < TJ / TS; ++s) {
@@ -594,7 +596,7 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
         b.part[3 * b.pstride + ii[k]] = ap[k];
     }
     if (stats && lane == 0) {
-        atomicAdd(&stats[0], (unsigned long long)(b.t1 - b.t0) * CHUNKS);
+        atomicAdd(&stats[0], (unsigned long long)((b.t1 - b.t0 + b.tstep - 1) / b.tstep) * CHUNKS);
         atomicAdd(&stats[1], (unsigned long long)nearv);
     }
 }
@@ -800,9 +802,15 @@ __device__ __forceinline__ Block single_block(const PairArgs &a)
     Block b;
     b.ibase = (int64_t)ib * WG * IPL;
     b.iend = a.ni;
-    tile_range(a, jp, b.t0, b.t1);
+    // partition jp takes every jsplit-th tile: the near tiles of a wave,
+    // neighbours along the curve, spread over the partitions instead of
+    // piling up in one
+    b.t0 = jp;
+    b.t1 = a.ntiles;
+    b.tstep = a.jsplit;
+    b.fbase = 0;
     b.fstride = a.ntiles;
-    b.flags = a.nearflag + (int64_t)ib * (WG / 64) * a.ntiles + b.t0;
+    b.flags = a.nearflag + (int64_t)ib * (WG / 64) * a.ntiles;
     b.part = a.partial + (int64_t)jp * 4 * a.ni;
     b.pstride = a.ni;
     return b;
@@ -969,6 +977,8 @@ __device__ __forceinline__ Block batch_block(const BatchArgs &a, int item)
     b.iend = st.pofs + st.n;
     b.t0 = st.tile0;
     b.t1 = st.tile0 + st.ntiles;
+    b.tstep = 1;
+    b.fbase = st.tile0;
     b.fstride = st.ntiles;
     b.flags = a.nearflag + st.flagofs + (int64_t)it.y * (WG / 64) * st.ntiles;
     b.part = a.partial;
