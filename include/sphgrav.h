@@ -159,14 +159,17 @@ int sg_set_batch_config(sg_context *ctx, int precision, int wg, int ipl);
 int sg_get_batch_config(sg_context *ctx, int precision, int *wg, int *ipl);
 
 /* HIP-event timings of the last completed evaluation, in milliseconds:
- * prep = validation, curve sort, gather and tile boxes; far = the far-field
- * pair kernel alone; near = near-field kernel plus the final reduction.
- * Any pointer may be NULL. */
+ * prep = validation, curve sort, gather and tile boxes; far = the pair
+ * kernel (in the FP32C forward pass it also takes the near chunks, listed
+ * per wave and run after its far loop); near = the separate near-field
+ * kernel where there is one (FP64 pass, grad-h pass, adjoint) plus the
+ * final reduction.  Any pointer may be NULL. */
 int sg_last_timings(sg_context *ctx, float *prep_ms, float *far_ms, float *near_ms);
 
 /* Statistics of the last evaluation: number of (wave, 64-particle j-chunk)
- * visits and how many of them took the near-field kernel (a chunk whose box
- * test cannot rule out softened pairs for the wave). */
+ * visits and how many of them were near (a chunk for which the box and
+ * per-particle tests cannot rule out softened pairs for the wave), i.e. took
+ * the near-field path. */
 int sg_last_tile_stats(sg_context *ctx, long long *visits, long long *near_visits);
 
 /* Evaluation that also forms the grad-h gravitational softening term of
