@@ -254,9 +254,9 @@ __device__ __forceinline__ void near_i_state(const SortedSet &I, const int64_t (
 constexpr int NLIST = 128;   // deferred near tiles per wave (more run at once)
 struct NearSlice {
     double4 p[TC];           // x, y, z relative to the origin; w: {FP32 m, FP32 H^2}
-    double m[TC];
     double hs[TC];           // signed H (> 0: gas)
-    int idx[TC];
+    double hinv[TC];         // 1 / |H|
+    double m[TC];
     uint32_t list[NLIST];    // (tile - t0) << 4 | chunk mask
 };
 
@@ -284,13 +284,12 @@ __device__ __forceinline__ void near_chunk_slice(
         const int64_t j = (int64_t)t * TJ + c * TC + lane;
         const double rx = J.x[j] - Ox, ry = J.y[j] - Oy, rz = J.z[j] - Oz, mj = J.m[j];
         const double hs = J.Hs[j];
-        const int id = J.idx[j];
         const float mh[2] = {(float)mj, (float)(hs * hs)};
         wave_lds_sync();
         sl.p[lane] = make_double4(rx, ry, rz, __builtin_bit_cast(double, mh));
-        sl.m[lane] = mj;
         sl.hs[lane] = hs;
-        sl.idx[lane] = id;
+        sl.hinv[lane] = 1.0 / fabs(hs);
+        sl.m[lane] = mj;
         wave_lds_sync();
     }
     float hi2f[IPL];
@@ -337,17 +336,20 @@ __device__ __forceinline__ void near_chunk_slice(
         jmask &= jmask - 1;
         const double4 p = sl.p[jj];
         const float2 mh = __builtin_bit_cast(float2, p.w);
-        const double mj = sl.m[jj], hs = sl.hs[jj], ha = fabs(hs);
-        const int idj = sl.idx[jj];
+        const double mj = sl.m[jj], hs = sl.hs[jj], ha = fabs(hs), hinv = sl.hinv[jj];
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
             const double ddx = p.x - xr[k], ddy = p.y - yr[k], ddz = p.z - zr[k];
             // the same FP32 decision as above
             const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            if (r2 < fmaxf(hi2f[k], mh.y))
-                near_pair<true, NK>(ddx, ddy, ddz, Hi[k], invHi[k], hi2[k], ha, 1.0 / ha, ha * ha,
-                                    mj, hs > 0.0, idj == idi[k], ax[k], ay[k], az[k], ap[k]);
+            if (r2 < fmaxf(hi2f[k], mh.y)) {
+                // the self pair only matters at r = 0: the index is read then
+                const bool same = ddx == 0.0 && ddy == 0.0 && ddz == 0.0 &&
+                                  J.idx[(int64_t)t * TJ + c * TC + jj] == idi[k];
+                near_pair<true, NK>(ddx, ddy, ddz, Hi[k], invHi[k], hi2[k], ha, hinv, ha * ha, mj,
+                                    hs > 0.0, same, ax[k], ay[k], az[k], ap[k]);
+            }
         }
     }
 }
