@@ -219,29 +219,29 @@ def test_far_path_dominates_with_small_h(grav, prec):
 
 
 @pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
-@pytest.mark.parametrize("wg,ipl", [(64, 1), (256, 1), (128, 2)])
-def test_waves_near_every_tile(grav, prec, wg, ipl):
+@pytest.mark.parametrize("wg,ipl,jsplit", [(64, 1, 1), (256, 1, 1), (128, 2, 1), (256, 1, 0)])
+def test_waves_near_every_tile(grav, prec, wg, ipl, jsplit):
     """40 gas particles with H = 100 (ten times the set's radius): every pair
     they take part in is softened, so their waves find every one of the 256
-    tiles near (more than the 128 tiles a wave's deferred near list holds,
-    which sends the rest through the list-overflow path), and their chunks
-    are near for every wave."""
+    tiles near and their chunks are near for every wave.  With one
+    j-partition a wave meets more near tiles than its deferred near list
+    holds (128), which sends the rest through the list-overflow path."""
     d = ics.plummer(65536, seed=21)
     rng = np.random.default_rng(5)
     big = rng.choice(65536, 40, replace=False)
     d["range2"][big] = 1.0e4
     old = grav.get_config(prec)
-    grav.set_config(prec, wg, ipl, 0)
+    grav.set_config(prec, wg, ipl, jsplit)
     try:
         res = grav.compute(*_args(d), nkernel=0, precision=prec)
         visits, nearv = grav.last_tile_stats()
     finally:
         grav.set_config(prec, *old)
-    print(f"[bigH {wg}x{ipl}] near chunk visits {nearv}/{visits}")
+    print(f"[bigH {wg}x{ipl}/{jsplit}] near chunk visits {nearv}/{visits}")
     assert nearv > 0
     rows = np.unique(np.concatenate([big, rng.choice(65536, 400, replace=False)]))
     ref = oracle.direct(*_args(d), nkernel=0, rows=rows)
-    _check(f"bigH {wg}x{ipl}", prec, [r[rows] for r in res], ref)
+    _check(f"bigH {wg}x{ipl}/{jsplit}", prec, [r[rows] for r in res], ref)
 
 
 @pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
