@@ -310,15 +310,17 @@ void k_adj_far(AdjArgs a)
 
     // the wave's box and largest support radius (the forward kernels' test)
     float bl[3] = {q.xh, q.yh, q.zh}, bh[3] = {q.xh, q.yh, q.zh};
-    float reach = 0.0f;
     for (int c = 0; c < 3; ++c) {
         bl[c] = wave_min(bl[c]);
         bh[c] = wave_max(bh[c]);
-        reach = fmaxf(reach, fmaxf(-bl[c], bh[c]));
     }
     const float hmax = wave_max((float)Hd);
     const float px[1] = {q.xh}, py[1] = {q.yh}, pz[1] = {q.zh}, ph[1] = {(float)Hd};
-    const float wide_d2 = 3.0f * (WIDE_K * reach) * (WIDE_K * reach);
+    // chunks nearer than WIDE_K times the wave's largest |x_i - O| take the
+    // hi/lo body
+    const float wide_d2 = WIDE_K * WIDE_K *
+                          wave_max(fmaf(q.xh, q.xh, fmaf(q.yh, q.yh, q.zh * q.zh))) *
+                          (1.0f + FAR_MARGIN);
     const bool active = uniform_i(ibase + wv * 64 < iend) != 0;
     unsigned char *flags = a.nearflag + ((int64_t)ib * (WG / 64) + wv) * a.ntiles;
 
@@ -330,7 +332,12 @@ void k_adj_far(AdjArgs a)
         const unsigned mask =
             active ? near_mask<1>(a.chunks, t, tm, d2, Ox, Oy, Oz, bl, bh, hmax, px, py, pz, ph)
                    : 0u;
-        const bool wide = uniform_i(d2 < wide_d2) != 0;
+        unsigned widem = 0;
+        if (!FP64 && uniform_i(d2 < wide_d2) != 0)
+            for (int c = 0; c < CHUNKS; ++c)
+                if (uniform_i(box_dist2(a.chunks[(int64_t)t * CHUNKS + c], Ox, Oy, Oz, bl, bh) <
+                              wide_d2) != 0)
+                    widem |= 1u << c;
         if (lane == 0) flags[t] = (unsigned char)mask;
         nearv += __builtin_popcount(mask);
         if (WG == 64 && mask == (1u << CHUNKS) - 1) continue;
@@ -360,7 +367,7 @@ void k_adj_far(AdjArgs a)
                 if ((mask >> c) & 1u) continue;   // k_adj_near
                 if (FP64)
                     adj_chunk_fp64(s_p64 + o, s_w64 + o, rx, ry, rz, mk, wkx, wky, wkz, uk, acc);
-                else if (wide)
+                else if ((widem >> c) & 1u)
                     adj_chunk_fp32<true>(s_p32 + o, s_w32 + o, s_lo32 + o, q, acc);
                 else
                     adj_chunk_fp32<false>(s_p32 + o, s_w32 + o, s_lo32 + o, q, acc);

@@ -11,13 +11,11 @@ namespace sg {
 
 constexpr float FAR_MARGIN = 1.0e-3f;   // relative guard on the box-distance test
 // Hi-only differences x_j,hi - x_i,hi carry a relative error of about
-// 2^-24 (|x_i - O| + |x_j - O| + r) / r <= 2^-24 (2 + 2 |x_i - O| / r), with
-// |x_i - O| <= sqrt(3) reach (reach = per-axis max over the wave).  Tiles
-// nearer than sqrt(3) WIDE_K reach form the difference in FP64 and round it
-// once, which bounds the hi-only ones by 2^-24 (2 + 2 / WIDE_K) = 6 ulp
-// (3.6e-7).  Waves that straddle a jump of the space-filling curve have a
-// large reach; with WIDE_K = 0.5 about 2% of tiles take the FP64-difference
-// body on the giant-star set.
+// 2^-24 (|x_i - O| + |x_j - O| + r) / r <= 2^-24 (2 + 2 |x_i - O| / r).
+// Chunks whose box is nearer to the wave's box than WIDE_K rho (rho = the
+// wave's largest |x_i - O|) form the difference in FP64 and round it once,
+// which bounds the hi-only ones by 2^-24 (2 + 2 / WIDE_K) = 6 ulp (3.6e-7).
+// Waves that straddle a jump of the space-filling curve have a large rho.
 constexpr float WIDE_K = 0.5f;
 
 __device__ __forceinline__ float uniform_f(float v)

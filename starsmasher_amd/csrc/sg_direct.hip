@@ -379,7 +379,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
     int64_t ii[IPL];
     bool valid[IPL];
     float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
-    float hmax = 0.0f;
+    float hmax = 0.0f, reach2 = 0.0f;
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
         const int64_t i = b.ibase + wv * WI + k * 64 + lane;
@@ -395,17 +395,16 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         bl[0] = fminf(bl[0], xh[k]); bl[1] = fminf(bl[1], yh[k]); bl[2] = fminf(bl[2], zh[k]);
         bh[0] = fmaxf(bh[0], xh[k]); bh[1] = fmaxf(bh[1], yh[k]); bh[2] = fmaxf(bh[2], zh[k]);
         hmax = fmaxf(hmax, hf[k]);
+        reach2 = fmaxf(reach2, fmaf(xh[k], xh[k], fmaf(yh[k], yh[k], zh[k] * zh[k])));
     }
-    float reach = 0.0f;   // max |x_i - O| over the wave, per axis
     for (int c = 0; c < 3; ++c) {
         bl[c] = wave_min(bl[c]);
         bh[c] = wave_max(bh[c]);
-        reach = fmaxf(reach, fmaxf(-bl[c], bh[c]));
     }
     hmax = wave_max(hmax);
-    // close tiles take FP64-formed differences (reach is per axis, so the
-    // Euclidean reach is at most sqrt(3) times it)
-    const float wide_d2 = 3.0f * (WIDE_K * reach) * (WIDE_K * reach);
+    // close chunks take FP64-formed differences: those nearer than WIDE_K
+    // times the wave's largest |x_i - O| (Euclidean)
+    const float wide_d2 = WIDE_K * WIDE_K * wave_max(reach2) * (1.0f + FAR_MARGIN);
 
     double ax[IPL], ay[IPL], az[IPL], ap[IPL];
 #pragma unroll
@@ -423,7 +422,13 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         const unsigned mask =
             active ? near_mask<IPL>(chunks, t, tm, d2, Ox, Oy, Oz, bl, bh, hmax, xh, yh, zh, hf)
                    : 0u;
-        const bool wide = uniform_i(d2 < wide_d2) != 0;
+        // chunks of a close tile that are themselves close
+        unsigned widem = 0;
+        if (uniform_i(d2 < wide_d2) != 0)
+            for (int c = 0; c < CHUNKS; ++c)
+                if (uniform_i(box_dist2(chunks[(int64_t)t * CHUNKS + c], Ox, Oy, Oz, bl, bh) <
+                              wide_d2) != 0)
+                    widem |= 1u << c;
         nearv += __builtin_popcount(mask);
         bool near_tile_now = false;   // a full list: this tile's near chunks run below
         if (!FUSED) {
@@ -461,7 +466,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
             for (int h = 0; h < TS / TC; ++h) {
                 const int c = s * (TS / TC) + h;
                 if ((mask >> c) & 1u) continue;
-                if (wide)
+                if ((widem >> c) & 1u)
                     far_chunk_fp32<IPL, true>(s_hi + h * TC, s_p + h * TC, xh, yh, zh, xr, yr, zr,
                                               ax, ay, az, ap);
                 else
