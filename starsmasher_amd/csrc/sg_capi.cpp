@@ -396,7 +396,12 @@ int compute_device_impl(sg_context *c, int n, int ibeg, int ni, const double *x,
     a.pot = pot;
     if ((rc = grow(&c->partial, &c->cap_part, (int64_t)a.jsplit * 4 * ni)) != SG_OK) return rc;
     a.partial = c->partial;
-    if ((rc = grow(&c->nearflag, &c->cap_flag, a.ngroups * ntiles)) != SG_OK) return rc;
+    // per-(wave, tile) near flags, N/64 x N/256 bytes: only the passes with a
+    // separate near kernel (FP64, grad-h) read them; the fused FP32C pass
+    // keeps its near list in LDS
+    const bool flagged = psi || precision != SG_PREC_FP32C;
+    if ((rc = grow(&c->nearflag, &c->cap_flag, flagged ? a.ngroups * ntiles : 1)) != SG_OK)
+        return rc;
     a.nearflag = c->nearflag;
     a.stats = c->stats;
     if (psi) {
