@@ -394,7 +394,32 @@ struct AdjNearLds {
     double4 p[ANEAR_WAVES][ACHUNK];   // x, y, z relative to the origin; m
     double4 w[ANEAR_WAVES][ACHUNK];   // wx, wy, wz, u
     double2 h[ANEAR_WAVES][ACHUNK];   // signed H, original index
+    float4 wf[ANEAR_WAVES][ACHUNK];   // FAST: wx, wy, wz, u in FP32
+    float2 mf[ANEAR_WAVES][ACHUNK];   // FAST: m, H^2 in FP32
 };
+
+// FAST: one Newtonian pair in FP32 on an FP64-formed difference, the far
+// kernel's body (gm holds -GM).
+__device__ __forceinline__ void adj_newton_fp32(float dx, float dy, float dz, float r2, float ml,
+                                                float4 wl, const Lane32 &q, float (&f)[8])
+{
+    const float ri = __builtin_amdgcn_rsqf(r2);
+    const float ri2 = ri * ri;
+    const float ri3 = ri2 * ri;
+    const float wkdr = fmaf(q.wx, dx, fmaf(q.wy, dy, q.wz * dz));
+    const float wldr = fmaf(wl.x, dx, fmaf(wl.y, dy, wl.z * dz));
+    const float t = fmaf(ml, wkdr, -q.m * wldr);
+    const float s = fmaf(ml, q.u, q.m * wl.w);
+    const float c = ri3 * fmaf(3.0f * t, ri2, -s);
+    f[0] = fmaf(ri3, wl.x, f[0]);
+    f[1] = fmaf(ri3, wl.y, f[1]);
+    f[2] = fmaf(ri3, wl.z, f[2]);
+    f[3] = fmaf(ri3, ml, f[3]);
+    f[4] = fmaf(c, dx, f[4]);
+    f[5] = fmaf(c, dy, f[5]);
+    f[6] = fmaf(c, dz, f[6]);
+    f[7] = fmaf(wl.w, ri, fmaf(ri3, wldr, f[7]));
+}
 
 // One workgroup of ANEAR_WAVES waves per 64-particle group and j-partition.
 // The waves deal the group's flagged tiles round-robin, stage them in
@@ -428,6 +453,14 @@ __global__ __launch_bounds__(64 * ANEAR_WAVES) void k_adj_near(AdjArgs a)
     const bool gask = Hs > 0.0;
     const double mk = P.m[i], wkx = a.wx[i], wky = a.wy[i], wkz = a.wz[i], uk = a.u[i];
     const int idk = P.idx[i];
+    Lane32 q;   // FAST: the i side in FP32 (positions unused)
+    q.xh = q.yh = q.zh = q.xl = q.yl = q.zl = 0.0f;
+    q.m = (float)mk;
+    q.wx = (float)wkx;
+    q.wy = (float)wky;
+    q.wz = (float)wkz;
+    q.u = (float)uk;
+    q.h2 = (float)(Hk * Hk);
 
     Acc9 g = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int cnt = 0;
@@ -442,14 +475,45 @@ __global__ __launch_bounds__(64 * ANEAR_WAVES) void k_adj_near(AdjArgs a)
             lds.p[w][lane] = make_double4(P.x[j] - Ox, P.y[j] - Oy, P.z[j] - Oz, P.m[j]);
             lds.w[w][lane] = make_double4(a.wx[j], a.wy[j], a.wz[j], a.u[j]);
             lds.h[w][lane] = make_double2(P.Hs[j], (double)P.idx[j]);
+            if (FAST) {
+                const double hj = P.Hs[j];
+                lds.wf[w][lane] = make_float4((float)a.wx[j], (float)a.wy[j], (float)a.wz[j],
+                                              (float)a.u[j]);
+                lds.mf[w][lane] = make_float2((float)P.m[j], (float)(hj * hj));
+            }
             wave_lds_sync();
-            for (int jj = 0; jj < ACHUNK; ++jj) {
-                const double4 p = lds.p[w][jj];
-                const double4 c = lds.w[w][jj];
-                const double2 h = lds.h[w][jj];
-                adj_pair64<FAST, NK>(p.x - xr, p.y - yr, p.z - zr, Hk, invHk, mk, wkx, wky,
-                                 wkz, uk, gask, fabs(h.x), p.w, c.x, c.y, c.z, c.w,
-                                 (int)h.y == idk, g);
+            if (FAST) {
+                // Newtonian pairs (r^2 clear of both supports) in FP32 with
+                // FP32 sums over the chunk; softened, self and coincident
+                // pairs and those within the margin of a support in FP64
+                float f[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                for (int jj = 0; jj < ACHUNK; ++jj) {
+                    const double4 p = lds.p[w][jj];
+                    const double ddx = p.x - xr, ddy = p.y - yr, ddz = p.z - zr;
+                    const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const float2 mh = lds.mf[w][jj];
+                    if (r2 > fmaxf(q.h2, mh.y) * (1.0f + FAR_MARGIN)) {
+                        adj_newton_fp32(dx, dy, dz, r2, mh.x, lds.wf[w][jj], q, f);
+                    } else {
+                        const double4 c = lds.w[w][jj];
+                        const double2 h = lds.h[w][jj];
+                        adj_pair64<true, NK>(ddx, ddy, ddz, Hk, invHk, mk, wkx, wky, wkz, uk,
+                                             gask, fabs(h.x), p.w, c.x, c.y, c.z, c.w,
+                                             (int)h.y == idk, g);
+                    }
+                }
+                for (int c = 0; c < 7; ++c) g[c] += (double)f[c];
+                g[7] -= (double)f[7];
+            } else {
+                for (int jj = 0; jj < ACHUNK; ++jj) {
+                    const double4 p = lds.p[w][jj];
+                    const double4 c = lds.w[w][jj];
+                    const double2 h = lds.h[w][jj];
+                    adj_pair64<false, NK>(p.x - xr, p.y - yr, p.z - zr, Hk, invHk, mk, wkx,
+                                          wky, wkz, uk, gask, fabs(h.x), p.w, c.x, c.y, c.z,
+                                          c.w, (int)h.y == idk, g);
+                }
             }
         }
     }
