@@ -331,25 +331,40 @@ __device__ __forceinline__ void near_chunk_slice(
             ap[k] -= (double)fp[k];
         }
     }
+    // The masked pairs in FP64.  A wave-wide walk over the j of jmask would
+    // run near_pair with about one lane in eight active, so the walk only
+    // repeats the FP32 decision to give each lane its own list, and each
+    // lane then runs its list (the wave takes as many steps as its longest).
+    uint64_t own[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) own[k] = 0;
     while (jmask) {
         const int jj = __builtin_ctzll(jmask);
         jmask &= jmask - 1;
         const double4 p = sl.p[jj];
         const float2 mh = __builtin_bit_cast(float2, p.w);
-        const double mj = sl.m[jj], hs = sl.hs[jj], ha = fabs(hs), hinv = sl.hinv[jj];
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
-            const double ddx = p.x - xr[k], ddy = p.y - yr[k], ddz = p.z - zr[k];
             // the same FP32 decision as above
-            const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
+            const float dx = (float)(p.x - xr[k]), dy = (float)(p.y - yr[k]),
+                        dz = (float)(p.z - zr[k]);
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            if (r2 < fmaxf(hi2f[k], mh.y)) {
-                // the self pair only matters at r = 0: the index is read then
-                const bool same = ddx == 0.0 && ddy == 0.0 && ddz == 0.0 &&
-                                  J.idx[(int64_t)t * TJ + c * TC + jj] == idi[k];
-                near_pair<true, NK>(ddx, ddy, ddz, Hi[k], invHi[k], hi2[k], ha, hinv, ha * ha, mj,
-                                    hs > 0.0, same, ax[k], ay[k], az[k], ap[k]);
-            }
+            if (r2 < fmaxf(hi2f[k], mh.y)) own[k] |= 1ull << jj;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        while (own[k]) {
+            const int jj = __builtin_ctzll(own[k]);
+            own[k] &= own[k] - 1;
+            const double4 p = sl.p[jj];
+            const double mj = sl.m[jj], hs = sl.hs[jj], ha = fabs(hs), hinv = sl.hinv[jj];
+            const double ddx = p.x - xr[k], ddy = p.y - yr[k], ddz = p.z - zr[k];
+            // the self pair only matters at r = 0: the index is read then
+            const bool same = ddx == 0.0 && ddy == 0.0 && ddz == 0.0 &&
+                              J.idx[(int64_t)t * TJ + c * TC + jj] == idi[k];
+            near_pair<true, NK>(ddx, ddy, ddz, Hi[k], invHi[k], hi2[k], ha, hinv, ha * ha, mj,
+                                hs > 0.0, same, ax[k], ay[k], az[k], ap[k]);
         }
     }
 }
