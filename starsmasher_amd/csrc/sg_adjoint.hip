@@ -399,11 +399,10 @@ struct AdjNearLds {
 };
 
 // FAST: one Newtonian pair in FP32 on an FP64-formed difference, the far
-// kernel's body (gm holds -GM).
-__device__ __forceinline__ void adj_newton_fp32(float dx, float dy, float dz, float r2, float ml,
+// kernel's body (gm holds -GM); ri = 0 adds nothing.
+__device__ __forceinline__ void adj_newton_fp32(float dx, float dy, float dz, float ri, float ml,
                                                 float4 wl, const Lane32 &q, float (&f)[8])
 {
-    const float ri = __builtin_amdgcn_rsqf(r2);
     const float ri2 = ri * ri;
     const float ri3 = ri2 * ri;
     const float wkdr = fmaf(q.wx, dx, fmaf(q.wy, dy, q.wz * dz));
@@ -484,27 +483,35 @@ __global__ __launch_bounds__(64 * ANEAR_WAVES) void k_adj_near(AdjArgs a)
             wave_lds_sync();
             if (FAST) {
                 // Newtonian pairs (r^2 clear of both supports) in FP32 with
-                // FP32 sums over the chunk; softened, self and coincident
-                // pairs and those within the margin of a support in FP64
+                // FP32 sums over the chunk, every lane at once (the others add
+                // zero); softened, self and coincident pairs and those within
+                // the margin of a support in FP64, each lane walking its own
+                // list of them
                 float f[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                uint64_t own = 0;
                 for (int jj = 0; jj < ACHUNK; ++jj) {
                     const double4 p = lds.p[w][jj];
-                    const double ddx = p.x - xr, ddy = p.y - yr, ddz = p.z - zr;
-                    const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
+                    const float dx = (float)(p.x - xr), dy = (float)(p.y - yr),
+                                dz = (float)(p.z - zr);
                     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                     const float2 mh = lds.mf[w][jj];
-                    if (r2 > fmaxf(q.h2, mh.y) * (1.0f + FAR_MARGIN)) {
-                        adj_newton_fp32(dx, dy, dz, r2, mh.x, lds.wf[w][jj], q, f);
-                    } else {
-                        const double4 c = lds.w[w][jj];
-                        const double2 h = lds.h[w][jj];
-                        adj_pair64<true, NK>(ddx, ddy, ddz, Hk, invHk, mk, wkx, wky, wkz, uk,
-                                             gask, fabs(h.x), p.w, c.x, c.y, c.z, c.w,
-                                             (int)h.y == idk, g);
-                    }
+                    const bool newton = r2 > fmaxf(q.h2, mh.y) * (1.0f + FAR_MARGIN);
+                    const float ri = newton ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+                    adj_newton_fp32(dx, dy, dz, ri, mh.x, lds.wf[w][jj], q, f);
+                    if (!newton) own |= 1ull << jj;
                 }
                 for (int c = 0; c < 7; ++c) g[c] += (double)f[c];
                 g[7] -= (double)f[7];
+                while (own) {
+                    const int jj = __builtin_ctzll(own);
+                    own &= own - 1;
+                    const double4 p = lds.p[w][jj];
+                    const double4 c = lds.w[w][jj];
+                    const double2 h = lds.h[w][jj];
+                    adj_pair64<true, NK>(p.x - xr, p.y - yr, p.z - zr, Hk, invHk, mk, wkx, wky,
+                                         wkz, uk, gask, fabs(h.x), p.w, c.x, c.y, c.z, c.w,
+                                         (int)h.y == idk, g);
+                }
             } else {
                 for (int jj = 0; jj < ACHUNK; ++jj) {
                     const double4 p = lds.p[w][jj];
