@@ -17,15 +17,17 @@
 //   * j data is staged through LDS per workgroup as FP32 {x_hi, y_hi, z_hi, m}
 //     and as FP64 {x, y, z} relative to an FP64 origin taken per workgroup,
 //     written from coalesced FP64 global loads and read back as same-address
-//     (broadcast) LDS reads.  Distant tiles use hi-only differences; tiles
-//     close to the wave form the difference in FP64 and round it once;
+//     (broadcast) LDS reads.  Chunks use hi-only differences, except those
+//     nearer to the wave's box than WIDE_K times its largest |x_i - O|,
+//     which form the difference in FP64 and round it once;
 //   * FP32 partial sums over FOLD j are folded into FP64 accumulators (the
 //     cross-tile and cross-partition reductions are FP64);
 //   * near chunks (pairs that may be softened) are listed per wave and run
 //     after its far loop in the wave's own LDS slice: FP64-formed
 //     differences, the FP32 body for the Newtonian lanes, FP64 near_pair for
-//     the softened ones (the Wendland polynomials carry coefficients up to
-//     9009 that cancel to O(1) near q = 1, which FP32 cannot resolve);
+//     the softened ones, each lane walking its own list of them (the
+//     Wendland polynomials carry coefficients up to 9009 that cancel to O(1)
+//     near q = 1, which FP32 cannot resolve);
 //   * the FP64 comparison pass and the grad-h pass flag near chunks for a
 //     separate near kernel instead;
 //   * REDUCE kernel: fixed-order FP64 sum over j-partitions, scattered back to
@@ -55,7 +57,7 @@ struct Block {
     int64_t iend;           // end of the i segment the block lies in
     int t0, t1, tstep;      // j-tiles t0, t0 + tstep, ... below t1
     int fbase;              // near flags are indexed by t - fbase
-    unsigned char *flags;   // near flags of the workgroup's first wave, [t - t0]
+    unsigned char *flags;   // near flags of the workgroup's first wave, [t - fbase]
     int64_t fstride;        // flag stride between consecutive waves
     double *part;           // partial sums: part[c * pstride + i], c = ax, ay, az, pot
     int64_t pstride;
@@ -268,9 +270,9 @@ struct NearSlice {
 // whose pair may be softened (r^2 < max(H_i^2, H_j^2) in FP32, r = 0
 // included) masked out; the j where a lane was masked are recorded in a
 // wave-uniform bit mask, and those lanes' pairs then run in FP64 through
-// near_pair from the slice (a pair is taken by exactly one of the two, so a
-// borderline FP32 decision costs only rounding).  Wave-local: no block
-// barrier.
+// near_pair from the slice, each lane over its own list (a pair is taken by
+// exactly one of the two, so a borderline FP32 decision costs only
+// rounding).  Wave-local: no block barrier.
 template <int IPL, int NK>
 __device__ __forceinline__ void near_chunk_slice(
     NearSlice &sl, const SortedSet &J, int t, int c, double Ox, double Oy, double Oz,
