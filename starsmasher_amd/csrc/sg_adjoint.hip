@@ -24,11 +24,14 @@
 // instructions and one v_rsq_f32.
 //
 // Layout and classification follow the forward pass: Morton-sorted
-// particles, 256-particle j-tiles, a per-(wave, tile) box test that sends
-// every tile with a possibly softened pair to the near kernel.  The far
-// kernel runs the Newtonian pair in FP32 on hi/lo positions relative to an
-// FP64 origin per workgroup, with FP32 sums over 32 j folded into FP64; the
-// near kernel (softened, self and coincident pairs) runs in FP64.
+// particles, 256-particle j-tiles of four 64-particle chunks, box and
+// per-lane tests (near_mask) that send every chunk with a possibly softened
+// pair to the near kernel.  The far kernel runs the Newtonian pair in FP32
+// on hi/lo positions relative to an FP64 origin per workgroup, with FP32
+// sums over 32 j folded into FP64.  The near kernel runs in FP64; in the
+// FP32C pass it takes the Newtonian pairs of its chunks in FP32 on
+// FP64-formed differences and only the softened, self and coincident pairs
+// in FP64.
 #include "sg_internal.h"
 #include "sg_device.h"
 
