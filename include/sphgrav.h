@@ -117,6 +117,14 @@ int sg_compute_device(sg_context *ctx, int n, int ibeg, int ni,
  * sg_compute_device (SG_ERR_MASS, SG_ERR_RANGE2, SG_ERR_NONFINITE) or SG_OK. */
 int sg_synchronize(sg_context *ctx);
 
+/* Gravitational potential energy epot = 1/2 sum_i m_i pot_i of device
+ * arrays m and pot (n >= 1), summed in FP64 in a fixed order (the same value
+ * for the same inputs on every call), written to the host double *epot.
+ * Replaces the host loop of output.f:232-252 when the outputs stay in HBM.
+ * Ordered on the context's stream after earlier work; blocks until done. */
+int sg_potential_energy(sg_context *ctx, int n, const double *m, const double *pot,
+                        double *epot);
+
 /* Milliseconds of the main pair kernel(s) of the last completed evaluation,
  * measured with HIP events on the context's stream (call after
  * sg_synchronize). */

@@ -34,7 +34,7 @@ EXPORTS = (
     "sg_get_batch_config", "sg_synchronize", "sg_last_kernel_ms", "sg_compute_host",
     "sg_set_config", "sg_get_config", "sg_last_timings", "sg_last_tile_stats", "sg_init_device",
     "sg_firsthalf", "sg_lasthalf", "sg_vjp_device", "sg_vjp_host", "sg_compute_psi_device",
-    "sg_compute_psi_host", "sg_psi_device", "sg_psi_host", "sg_grav_psi",
+    "sg_compute_psi_host", "sg_psi_device", "sg_psi_host", "sg_grav_psi", "sg_potential_energy",
 )
 
 
@@ -103,12 +103,13 @@ def load() -> C.CDLL:
     L.sg_grav_psi.argtypes = [C.c_int, C.c_int, C.c_int] + [_dp] * 5 + [C.c_int, _dp]
     L.sg_vjp_device.argtypes = [_vp, C.c_int] + [_vp] * 5 + [C.c_int, C.c_int] + [_vp] * 9
     L.sg_vjp_host.argtypes = [_vp, C.c_int] + [_dp] * 5 + [C.c_int, C.c_int] + [_dp] * 9
+    L.sg_potential_energy.argtypes = [_vp, C.c_int, _vp, _vp, C.POINTER(C.c_double)]
     for name in ("sg_last_status", "sg_device_count", "sg_create", "sg_destroy", "sg_compute_device",
                  "sg_compute_host", "sg_synchronize", "sg_last_kernel_ms", "sg_set_config",
                  "sg_get_config", "sg_last_timings", "sg_last_tile_stats", "sg_init_device",
                  "sg_firsthalf", "sg_lasthalf", "sg_vjp_device", "sg_vjp_host",
                  "sg_compute_psi_device", "sg_compute_psi_host", "sg_psi_device",
-                 "sg_psi_host", "sg_grav_psi"):
+                 "sg_psi_host", "sg_grav_psi", "sg_potential_energy"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

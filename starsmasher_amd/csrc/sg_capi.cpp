@@ -57,6 +57,8 @@ struct sg_context {
     int64_t cap_vin = 0, cap_vout = 0;
     double *psi_out = nullptr;          // host-path staging of the grad-h term
     int64_t cap_psi_out = 0;
+    double *epot_work = nullptr;        // block sums and result of sg_potential_energy
+    int64_t cap_epot = 0;
 
     // tile shape per precision (autotuned on the N = 2^20 giant-star set)
     int wg[2] = {256, 256};
@@ -294,7 +296,7 @@ int sg_destroy(sg_context *c)
                     c->scot[0], c->scot[1], c->scot[2], c->scot[3],
                     c->vin[0], c->vin[1], c->vin[2], c->vin[3],
                     c->vout[0], c->vout[1], c->vout[2], c->vout[3], c->vout[4], c->psipart,
-                    c->psi_out};
+                    c->psi_out, c->epot_work};
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -703,6 +705,21 @@ int sg_synchronize(sg_context *c)
     if (err & sg::ERR_NONFINITE) return SG_ERR_NONFINITE;
     if (err & sg::ERR_MASS) return SG_ERR_MASS;
     if (err & sg::ERR_RANGE2) return SG_ERR_RANGE2;
+    return SG_OK;
+}
+
+int sg_potential_energy(sg_context *c, int n, const double *m, const double *pot,
+                        double *epot)
+{
+    if (!c || n < 1 || !m || !pot || !epot) return SG_ERR_ARG;
+    HIP_TRY(hipSetDevice(c->device));
+    constexpr int NWORK = 1025;   // up to 1024 block sums, then the result
+    int rc;
+    if ((rc = grow(&c->epot_work, &c->cap_epot, NWORK)) != SG_OK) return rc;
+    HIP_TRY(sg::launch_potential_energy(m, pot, n, c->epot_work, NWORK, c->stream));
+    HIP_TRY(hipMemcpyAsync(epot, c->epot_work + NWORK - 1, sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return SG_OK;
 }
 

@@ -168,6 +168,9 @@ hipError_t launch_tile_meta(const double *sx, const double *sy, const double *sz
                             const double *sH, int ntiles, TileMeta *tiles, TileMeta *chunks,
                             hipStream_t s);
 hipError_t launch_fill_zero_int(int *p, int n, hipStream_t s);
+// 1/2 sum m pot in FP64 into work[nwork - 1] (work holds nwork doubles)
+hipError_t launch_potential_energy(const double *m, const double *pot, int n, double *work,
+                                   int nwork, hipStream_t s);
 // Batch preparation: keys = (set << 42) | top 42 bits of the Morton code, and
 // the gather into the padded per-set layout.
 hipError_t launch_morton_batch(const double *x, const double *y, const double *z, int n,

@@ -92,6 +92,43 @@ __global__ __launch_bounds__(RB) void k_bbox_final(const double *__restrict__ re
         for (int c = 0; c < 6; ++c) bbox[c] = v[c];
 }
 
+// Gravitational potential energy, the diagnostic of output.f:232-252:
+// epot = 1/2 sum_i m_i phi_i, in FP64.  Each block sums a fixed contiguous
+// range with a fixed tree, and one block sums the block results in order, so
+// the value does not depend on scheduling.
+__device__ inline double block_sum(double v)
+{
+    __shared__ double sh[RB / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int ww = 0; ww < RB / 64; ++ww) t += sh[ww];
+    return t;
+}
+
+__global__ __launch_bounds__(RB) void k_epot_part(const double *__restrict__ m,
+                                                  const double *__restrict__ pot, int n, int per,
+                                                  double *__restrict__ part)
+{
+    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min<int64_t>(b0 + per, n);
+    double v = 0.0;
+    for (int64_t k = b0 + threadIdx.x; k < b1; k += RB) v = fma(m[k], pot[k], v);
+    v = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(RB) void k_epot_final(const double *__restrict__ part, int nblocks,
+                                                   double *__restrict__ out)
+{
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += RB) v += part[b];
+    v = block_sum(v);
+    if (threadIdx.x == 0) out[0] = 0.5 * v;
+}
+
 // Spread the low 21 bits of v so that bit b lands at bit 3b (bit interleave
 // by successive shift-and-mask; public-domain bit-twiddling construction).
 __device__ inline uint64_t spread3(uint64_t v)
@@ -315,6 +352,20 @@ hipError_t launch_gather_batch(const double *x, const double *y, const double *z
     if (padded <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather_batch, dim3(nblocks_for(padded)), dim3(RB), 0, s, x, y, z, m, r2,
                        perm, offsets, sets, nsets, padded, ox, oy, oz, om, oH, oidx);
+    return hipGetLastError();
+}
+
+hipError_t launch_potential_energy(const double *m, const double *pot, int n, double *work,
+                                   int nwork, hipStream_t s)
+{
+    // work: nwork - 1 block partials, then the result
+    if (n < 1 || nwork < 2) return hipErrorInvalidValue;
+    const int nb = std::min(nwork - 1, std::max(1, (n + 4 * RB - 1) / (4 * RB)));
+    const int per = (n + nb - 1) / nb;
+    hipLaunchKernelGGL(k_epot_part, dim3(nb), dim3(RB), 0, s, m, pot, n, per, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_epot_final, dim3(1), dim3(RB), 0, s, work, nb, work + nwork - 1);
     return hipGetLastError();
 }
 

@@ -320,6 +320,16 @@ class Gravity:
     def synchronize(self):
         check(self._L.sg_synchronize(self._h), "sg_synchronize")
 
+    def potential_energy(self, n, m, pot) -> float:
+        """epot = 1/2 sum m pot of device arrays (torch tensors or addresses),
+        summed in FP64 in a fixed order on the device: the energy diagnostic
+        of output.f:232-252 without copying the potentials to the host."""
+        n = int(n)
+        mp, pp = _dev_ptr(m, "m", n), _dev_ptr(pot, "pot", n)
+        e = C.c_double()
+        check(self._L.sg_potential_energy(self._h, n, mp, pp, C.byref(e)), "sg_potential_energy")
+        return e.value
+
     def last_kernel_ms(self) -> float:
         ms = C.c_float()
         check(self._L.sg_last_kernel_ms(self._h, C.byref(ms)), "sg_last_kernel_ms")

@@ -8,6 +8,7 @@ Golden potentials stored by the reference in its dumps: <= 4e-7 (their own
 FP32 floor), both precisions.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -181,6 +182,41 @@ def test_device_validation_errors(grav):
         with pytest.raises(_lib.SphgravError) as e:
             grav.synchronize()
         assert e.value.status == status
+
+
+@pytest.mark.parametrize("n", [1, 1000, 262_145, 2_000_003])
+def test_potential_energy_diagnostic(grav, n):
+    """epot = 1/2 sum m phi (output.f:232-252) summed on the device in FP64:
+    against the correctly rounded sum of the products (math.fsum; the
+    products' own rounding is below 1.2e-16 relative with terms of one sign)
+    and the oracle's loop, for sizes from one particle to beyond the 1024
+    block sums, and bit-for-bit the same on a second call."""
+    rng = np.random.default_rng(n)
+    m = rng.uniform(1e-6, 1.0, n)
+    pot = -rng.uniform(0.1, 10.0, n)
+    dm, dp = DeviceArray.from_host(m), DeviceArray.from_host(pot)
+    e1 = grav.potential_energy(n, dm, dp)
+    e2 = grav.potential_energy(n, dm, dp)
+    assert e1 == e2
+    exact = 0.5 * math.fsum((m * pot).tolist())
+    assert abs(e1 - exact) <= 1e-13 * abs(exact)
+    assert abs(e1 - oracle.epot(m, pot)) <= 1e-12 * abs(exact)
+
+
+def test_potential_energy_of_gravity_outputs(grav, plummer16k):
+    """The diagnostic on the potentials the pair kernel leaves in HBM."""
+    d, _ = plummer16k
+    n = d["x"].size
+    dev = [DeviceArray.from_host(d[k]) for k in ("x", "y", "z", "m", "range2")]
+    out = [DeviceArray(n) for _ in range(4)]
+    grav.compute_device(n, *dev, *out, nkernel=0, precision=SG_PREC_FP64)
+    grav.synchronize()
+    e = grav.potential_energy(n, dev[3], out[3])
+    want = oracle.epot(d["m"], out[3].download())
+    assert e < 0.0 and abs(e - want) <= 1e-12 * abs(want)
+    with pytest.raises(_lib.SphgravError) as err:
+        grav.potential_energy(0, dev[3], out[3])
+    assert err.value.status == _lib.SG_ERR_ARG
 
 
 def test_deterministic(grav, plummer16k):
