@@ -718,7 +718,11 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
             if (FAST) {
                 // Newtonian pairs in FP32 (hi+lo differences, sums folded into
                 // FP64 every FOLD j); the pairs with r^2 < max(H_i^2, H_j^2),
-                // including r = 0, take the FP64 pair on their lanes only.
+                // including r = 0, go on their lane's list and then take the
+                // FP64 pair, each lane walking its own list.
+                uint64_t own[IPL];
+#pragma unroll
+                for (int k = 0; k < IPL; ++k) own[k] = 0;
                 for (int f0 = 0; f0 < NEAR_CHUNK; f0 += FOLD) {
                     float fx[IPL], fy[IPL], fz[IPL], fp[IPL];
 #pragma unroll
@@ -741,15 +745,7 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
                             fy[k] = fmaf(mr3, dy, fy[k]);
                             fz[k] = fmaf(mr3, dz, fz[k]);
                             fp[k] += mr;
-                            if (soft) {
-                                const double4 p = lds.p[w][jj];
-                                const double4 h = lds.h[w][jj];
-                                near_pair<FAST, NK, PSI>(p.x - xr[k], p.y - yr[k], p.z - zr[k],
-                                                         Hi[k], invHi[k], hi2[k], fabs(h.x), h.z,
-                                                         h.w, p.w, h.x > 0.0,
-                                                         (int)h.y == idi[k], ax[k], ay[k], az[k],
-                                                         ap[k], &ps[k]);
-                            }
+                            if (soft) own[k] |= 1ull << jj;
                         }
                     }
 #pragma unroll
@@ -758,6 +754,19 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
                         ay[k] += (double)fy[k];
                         az[k] += (double)fz[k];
                         ap[k] -= (double)fp[k];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < IPL; ++k) {
+                    while (own[k]) {
+                        const int jj = __builtin_ctzll(own[k]);
+                        own[k] &= own[k] - 1;
+                        const double4 p = lds.p[w][jj];
+                        const double4 h = lds.h[w][jj];
+                        near_pair<FAST, NK, PSI>(p.x - xr[k], p.y - yr[k], p.z - zr[k], Hi[k],
+                                                 invHi[k], hi2[k], fabs(h.x), h.z, h.w, p.w,
+                                                 h.x > 0.0, (int)h.y == idi[k], ax[k], ay[k],
+                                                 az[k], ap[k], &ps[k]);
                     }
                 }
             } else {
