@@ -67,6 +67,18 @@ def test_batch_matches_oracle(grav, prec, graph):
         _check(f"batch set{k} n={d['x'].size} prec={prec} graph={graph}", prec, r, ref)
 
 
+@pytest.mark.parametrize("nk", [1, 2])
+def test_batch_wendland_kernels(grav, nk):
+    """The batch kernels' Wendland W3,3 and C4 instantiations, with a point
+    particle, against the FP64 oracle in both precisions."""
+    sets = _sets([1500, 2049, 700], seed=11)
+    for prec in (SG_PREC_FP32C, SG_PREC_FP64):
+        res = grav.compute_batch(sets, nkernel=nk, precision=prec)
+        for k, (d, r) in enumerate(zip(sets, res)):
+            ref = oracle.direct(d["x"], d["y"], d["z"], d["m"], d["range2"], nkernel=nk)
+            _check(f"batch nk={nk} set{k} n={d['x'].size} prec={prec}", prec, r, ref)
+
+
 def test_batch_matches_single_launches(grav):
     sets = _sets([3000, 8000, 1200], seed=5)
     for prec in (SG_PREC_FP32C, SG_PREC_FP64):
