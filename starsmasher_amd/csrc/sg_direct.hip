@@ -641,12 +641,11 @@ __device__ __forceinline__ bool group_has_near(const Group &g)
 constexpr int NEAR_CHUNK = TC;   // j per LDS chunk of a near wave (one per lane)
 static_assert(NEAR_CHUNK == 64, "one staged particle per lane");
 
-// 24 KB per workgroup, so LDS does not cap occupancy below the register limit
+// 18 KB per workgroup, so LDS does not cap occupancy below the register limit
 struct NearLds {
     double4 p[NEAR_WAVES][NEAR_CHUNK];   // x, y, z relative to the origin; m
     double4 h[NEAR_WAVES][NEAR_CHUNK];   // signed H, original index (as double), 1/|H|, H^2
-    float4 f[NEAR_WAVES][NEAR_CHUNK];    // FP32 x_hi, y_hi, z_hi, m (FAST kernels)
-    float4 l[NEAR_WAVES][NEAR_CHUNK];    // FP32 x_lo, y_lo, z_lo, H^2 (FAST kernels)
+    float2 mh[NEAR_WAVES][NEAR_CHUNK];   // FP32 m, H^2 (FAST kernels)
 };
 
 // With PSI (IPL = 1 only) it also forms the grad-h softening term
@@ -682,18 +681,10 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
         ax[k] = ay[k] = az[k] = ap[k] = ps[k] = 0.0;
     }
 
-    // FP32 hi/lo of the i positions (the FAST kernels' Newtonian pairs)
-    float xh[IPL], yh[IPL], zh[IPL], xl[IPL], yl[IPL], zl[IPL], hi2f[IPL];
+    // FP32 H_i^2 for the FAST kernels' softened-pair test
+    float hi2f[IPL];
 #pragma unroll
-    for (int k = 0; k < IPL; ++k) {
-        xh[k] = (float)xr[k];
-        yh[k] = (float)yr[k];
-        zh[k] = (float)zr[k];
-        xl[k] = (float)(xr[k] - (double)xh[k]);
-        yl[k] = (float)(yr[k] - (double)yh[k]);
-        zl[k] = (float)(zr[k] - (double)zh[k]);
-        hi2f[k] = (float)hi2[k];
-    }
+    for (int k = 0; k < IPL; ++k) hi2f[k] = (float)hi2[k];
 
     int cnt = 0;
     for (int t = g.t0; t < g.t1; ++t) {
@@ -708,15 +699,11 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
             lds.p[w][lane] = make_double4(jx, jy, jz, J.m[j]);
             const double hs = J.Hs[j], ha = fabs(hs);
             lds.h[w][lane] = make_double4(hs, (double)J.idx[j], 1.0 / ha, ha * ha);
-            if (FAST) {
-                const float hx = (float)jx, hy = (float)jy, hz = (float)jz;
-                lds.f[w][lane] = make_float4(hx, hy, hz, (float)J.m[j]);
-                lds.l[w][lane] = make_float4((float)(jx - (double)hx), (float)(jy - (double)hy),
-                                             (float)(jz - (double)hz), (float)(ha * ha));
-            }
+            if (FAST) lds.mh[w][lane] = make_float2((float)J.m[j], (float)(ha * ha));
             wave_lds_sync();
             if (FAST) {
-                // Newtonian pairs in FP32 (hi+lo differences, sums folded into
+                // Newtonian pairs in FP32 (differences formed in FP64 and
+                // rounded once, as in the forward near tail; sums folded into
                 // FP64 every FOLD j); the pairs with r^2 < max(H_i^2, H_j^2),
                 // including r = 0, go on their lane's list and then take the
                 // FP64 pair, each lane walking its own list.
@@ -729,17 +716,17 @@ __device__ __forceinline__ void near_group(const SortedSet &I, const SortedSet &
                     for (int k = 0; k < IPL; ++k) fx[k] = fy[k] = fz[k] = fp[k] = 0.0f;
 #pragma unroll 2
                     for (int jj = f0; jj < f0 + FOLD; ++jj) {
-                        const float4 a = lds.f[w][jj];
-                        const float4 b = lds.l[w][jj];
+                        const double4 pd = lds.p[w][jj];
+                        const float2 mh = lds.mh[w][jj];   // m_j, H_j^2
 #pragma unroll
                         for (int k = 0; k < IPL; ++k) {
-                            const float dx = (a.x - xh[k]) + (b.x - xl[k]);
-                            const float dy = (a.y - yh[k]) + (b.y - yl[k]);
-                            const float dz = (a.z - zh[k]) + (b.z - zl[k]);
+                            const float dx = (float)(pd.x - xr[k]);
+                            const float dy = (float)(pd.y - yr[k]);
+                            const float dz = (float)(pd.z - zr[k]);
                             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                            const bool soft = r2 < fmaxf(hi2f[k], b.w);
+                            const bool soft = r2 < fmaxf(hi2f[k], mh.y);
                             const float ri = soft ? 0.0f : __builtin_amdgcn_rsqf(r2);
-                            const float mr = a.w * ri;
+                            const float mr = mh.x * ri;
                             const float mr3 = mr * ri * ri;
                             fx[k] = fmaf(mr3, dx, fx[k]);
                             fy[k] = fmaf(mr3, dy, fy[k]);
