@@ -15,6 +15,7 @@
  *   pbe/src/main.f:297-349                             interaction-balanced i split
  *   pbe/src/kernels.f:215-279                          dphi/dh closed forms (adjoint check, psi)
  *   pbe/src/advance.f90:716-737                        grad-h softening sum psi_i
+ *   pbe/src/output.f:232-252                           energy diagnostic epot = sum m phi / 2
  *
  * Pinning (what checks each function; see tests/test_oracle.py):
  *   sgo_direct        golden grpot of the reference's own start dumps (cubic),
@@ -33,6 +34,7 @@
  *   sgo_psi_rows      kernels.f's dphidh closed forms (tests restate them and
  *                     tie them to P and A: dphidh = -(P(q) + q P'(q))/2), the
  *                     self term dphidh(0) = c, and the table lookup's bound.
+ *   sgo_epot          math.fsum, the correctly rounded sum of the products.
  */
 #include <math.h>
 #include <stdint.h>

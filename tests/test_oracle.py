@@ -370,3 +370,16 @@ def test_psi_self_term_and_table():
     a = oracle.psi(pos[:, 0], pos[:, 1], pos[:, 2], m, r2, tabulated=False)
     b = oracle.psi(pos[:, 0], pos[:, 1], pos[:, 2], m, r2, tabulated=True)
     assert np.max(np.abs(a - b) / np.abs(a)) < 1e-6
+
+
+def test_epot_matches_correctly_rounded_sum():
+    """sgo_epot (output.f:232-252: epot = 1/2 sum m phi) against math.fsum of
+    the same products; a dropped factor 1/2 or a wrong array fails."""
+    import math
+    rng = np.random.default_rng(21)
+    m = rng.uniform(1e-6, 1.0, 50_001)
+    pot = -rng.uniform(0.1, 10.0, 50_001)
+    want = 0.5 * math.fsum((m * pot).tolist())
+    got = oracle.epot(m, pot)
+    assert abs(got - want) <= 1e-12 * abs(want)
+    assert oracle.epot(m[:1], pot[:1]) == 0.5 * m[0] * pot[0]
