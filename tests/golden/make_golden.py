@@ -11,6 +11,11 @@ It writes data only -- inputs and expected outputs:
   pbe/src/output.f:53-92 (Fortran sequential records: header, one record per
   particle of 16 x f64 + i32 + f64 [+ 2 x f64 when ncooling != 0], trailer).
 
+* ``collision_run.npz``: the short collision run of the whole StarSmasher
+  program (collision_case.py) as the reference program with its own CPU
+  gravity writes it: its start dump and the expected potentials, positions
+  and velocities of its two output dumps.
+
 * ``twin_vectors.npz``: outputs of the reference's own CPU twin
   ``get_gravity_using_cpus`` (pbe/src/cpu_grav.f) compiled from its source
   into oracle/_ref (``make -C oracle ref``), on small seeded inputs for
@@ -60,19 +65,26 @@ def read_dump(path: str) -> dict:
     if rec_len not in (140, 156):
         raise ValueError(f"{path}: unexpected particle record of {rec_len} bytes")
     f64 = np.frombuffer(b"".join(p[:120] for p in parts), dtype="<f8").reshape(ntot, 15)
+    # header: ntot, nnopt (i4); hco, hfloor, sep0, tf, dtout (f8); nout, nit (i4); t (f8)
+    (t,) = struct.unpack_from("<d", recs[0], 56)
     return {
         "x": f64[:, 0].copy(), "y": f64[:, 1].copy(), "z": f64[:, 2].copy(),
         "am": f64[:, 3].copy(), "hp": f64[:, 4].copy(), "u": f64[:, 12].copy(),
         "grpot": f64[:, 14].copy(), "header_bytes": np.int64(len(recs[0])),
+        "vx": f64[:, 6].copy(), "vy": f64[:, 7].copy(), "vz": f64[:, 8].copy(),
+        "rho": f64[:, 5].copy(), "t": np.float64(t),
     }
+
+
+DUMP_KEYS = ("x", "y", "z", "am", "hp", "u", "grpot", "header_bytes")
 
 
 def make_dumps() -> None:
     out = {}
     for name, rel in DUMPS:
         d = read_dump(os.path.join(REF, rel))
-        for k, v in d.items():
-            out[f"{name}/{k}"] = v
+        for k in DUMP_KEYS:
+            out[f"{name}/{k}"] = d[k]
         print(f"{rel}: N={d['x'].size} header={int(d['header_bytes'])} B")
     np.savez_compressed(os.path.join(HERE, "dumps.npz"), **out)
 
@@ -114,6 +126,41 @@ def make_twin_vectors() -> None:
     np.savez_compressed(os.path.join(HERE, "twin_vectors.npz"), **out)
 
 
+def make_collision_run() -> None:
+    """collision_run.npz: the start dump of the short collision run
+    (collision_case.py) and what the reference program, built with its own
+    CPU gravity (``make -C oracle ref-app``), writes for it on 2 MPI ranks:
+    the potentials of out0000.sph (the first gravity call) and positions,
+    velocities and potentials of out0001.sph (about five steps later)."""
+    import subprocess
+    import tempfile
+
+    from collision_case import write_case
+
+    subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref-app"], check=True)
+    exe = os.path.join(REPO, "oracle", "_ref", "starsmasher_cpu")
+    start = np.fromfile(os.path.join(REF, "example_input/collision/sph.start2u"), dtype=np.uint8)
+    out = {"start_bytes": start}
+    with tempfile.TemporaryDirectory() as tmp:
+        write_case(tmp, start, ngravprocs=2)
+        subprocess.run(["/opt/conda/bin/mpiexec", "-np", "2", exe], cwd=tmp, check=True,
+                       stdout=subprocess.DEVNULL, timeout=600)
+        d0 = read_dump(os.path.join(tmp, "out0000.sph"))
+        d1 = read_dump(os.path.join(tmp, "out0001.sph"))
+    out["out0000/grpot"] = d0["grpot"]
+    out["out0000/t"] = d0["t"]
+    for k in ("x", "y", "z", "vx", "vy", "vz", "grpot", "t"):
+        out[f"out0001/{k}"] = d1[k]
+    np.savez_compressed(os.path.join(HERE, "collision_run.npz"), **out)
+    print(f"collision run: N={d1['x'].size}, out0001 at t={float(d1['t']):.6f}")
+
+
 if __name__ == "__main__":
-    make_dumps()
-    make_twin_vectors()
+    sys.path.insert(0, HERE)
+    which = sys.argv[1:] or ["dumps", "twin", "collision"]
+    if "dumps" in which:
+        make_dumps()
+    if "twin" in which:
+        make_twin_vectors()
+    if "collision" in which:
+        make_collision_run()
