@@ -180,6 +180,18 @@ int sg_last_timings(sg_context *ctx, float *prep_ms, float *far_ms, float *near_
  * the near-field path. */
 int sg_last_tile_stats(sg_context *ctx, long long *visits, long long *near_visits);
 
+/* The same counts and two more of the FP32C pass (no reference
+ * counterpart; they state where FP64 arithmetic enters the pair loop):
+ * wide_visits = far (wave, chunk) visits whose pair differences were formed
+ * in FP64 and rounded once (chunks close to the wave relative to its spread
+ * about the workgroup origin), and fp64_pairs = (i, j) pairs evaluated
+ * entirely in FP64 (those that may be softened, r < max(H_i, H_j), the
+ * self pair included).  A chunk visit covers 64 j for each i of a wave
+ * (64 per lane and i-particle per lane).  Zero for the FP64 pass, whose
+ * pairs are all FP64.  Any pointer may be NULL. */
+int sg_last_precision_stats(sg_context *ctx, long long *visits, long long *near_visits,
+                            long long *wide_visits, long long *fp64_pairs);
+
 /* Evaluation that also forms the grad-h gravitational softening term of
  * each gas particle i of the i-range (range2_i > 0; 0 for point particles):
  *     psi_i = hp_i^-2 sum_{j: r_ij < 2 hp_i} m_j dphidh(r_ij / hp_i)

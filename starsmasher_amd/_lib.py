@@ -35,6 +35,7 @@ EXPORTS = (
     "sg_set_config", "sg_get_config", "sg_last_timings", "sg_last_tile_stats", "sg_init_device",
     "sg_firsthalf", "sg_lasthalf", "sg_vjp_device", "sg_vjp_host", "sg_compute_psi_device",
     "sg_compute_psi_host", "sg_psi_device", "sg_psi_host", "sg_grav_psi", "sg_potential_energy",
+    "sg_last_precision_stats",
 )
 
 
@@ -91,6 +92,7 @@ def load() -> C.CDLL:
     L.sg_get_config.argtypes = [_vp, C.c_int, _ip, _ip, _ip]
     L.sg_last_timings.argtypes = [_vp] + [C.POINTER(C.c_float)] * 3
     L.sg_last_tile_stats.argtypes = [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.sg_last_precision_stats.argtypes = [_vp] + [C.POINTER(C.c_longlong)] * 4
     L.sg_init_device.argtypes = [C.c_int]
     L.sg_firsthalf.argtypes = [C.c_int, C.c_int, C.c_int] + [_dp] * 5 + [C.c_int]
     L.sg_lasthalf.argtypes = [_dp] * 4
@@ -109,7 +111,7 @@ def load() -> C.CDLL:
                  "sg_get_config", "sg_last_timings", "sg_last_tile_stats", "sg_init_device",
                  "sg_firsthalf", "sg_lasthalf", "sg_vjp_device", "sg_vjp_host",
                  "sg_compute_psi_device", "sg_compute_psi_host", "sg_psi_device",
-                 "sg_psi_host", "sg_grav_psi", "sg_potential_energy"):
+                 "sg_psi_host", "sg_grav_psi", "sg_potential_energy", "sg_last_precision_stats"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -46,7 +46,9 @@ struct sg_context {
     int64_t cap_psipart = 0;
     unsigned char *nearflag = nullptr;
     int64_t cap_flag = 0;
-    unsigned long long *stats = nullptr;   // [2]
+    // [4]: chunk visits, near chunk visits, wide chunk visits (FP64-formed
+    // differences in the far loop), pairs evaluated in FP64 (FP32C pass)
+    unsigned long long *stats = nullptr;
     double *in[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double *out[4] = {nullptr, nullptr, nullptr, nullptr};
     // adjoint: sorted cotangents; host-path staging of cotangents and gradients
@@ -158,7 +160,7 @@ int ensure_capacity(sg_context *c, int64_t n, int64_t padded, int64_t ni)
         cap = 0;
         if ((rc = grow(&c->pb.err, &cap, 4)) != SG_OK) return rc;
         cap = 0;
-        if ((rc = grow(&c->stats, &cap, 2)) != SG_OK) return rc;
+        if ((rc = grow(&c->stats, &cap, 4)) != SG_OK) return rc;
     }
     if (padded > c->cap_pad || !c->pb.sx) {
         int64_t cap = 0;
@@ -412,7 +414,7 @@ int compute_device_impl(sg_context *c, int n, int ibeg, int ni, const double *x,
         a.psi = psi;
         a.psipart = c->psipart;
     }
-    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 4, s));
+    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 8, s));
 
     HIP_TRY(hipEventRecord(c->ev_k0, s));
     HIP_TRY(sg::launch_pair_far(precision, a, wg, s));
@@ -483,6 +485,9 @@ int sg_psi_device(sg_context *c, int n, int ibeg, int ni, const double *x, const
     a.jsplit = 1;
     a.ngroups = (ni + 63) / 64;
     a.psi = psi;
+    // the psi-only pass classifies no chunks: its statistics read zero
+    // rather than the previous evaluation's counts
+    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 8, s));
     HIP_TRY(hipEventRecord(c->ev_k0, s));
     HIP_TRY(hipEventRecord(c->ev_kf, s));
     HIP_TRY(sg::launch_psi_only(a, s));
@@ -671,7 +676,7 @@ int sg_compute_batch_device(sg_context *c, int nsets, const long long *offsets, 
             c->gexec = nullptr;
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 4, s));
+            HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 8, s));
             const hipError_t le = sg::launch_batch(precision, a, wg, ipl, nwg, s);
             const hipError_t ee = hipStreamEndCapture(s, &graph);
             HIP_TRY(le);
@@ -685,7 +690,7 @@ int sg_compute_batch_device(sg_context *c, int nsets, const long long *offsets, 
         }
         HIP_TRY(hipGraphLaunch(c->gexec, s));
     } else {
-        HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 4, s));
+        HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 8, s));
         HIP_TRY(sg::launch_batch(precision, a, wg, ipl, nwg, s));
     }
     HIP_TRY(hipEventRecord(c->ev_kf, s));
@@ -753,14 +758,22 @@ int sg_last_timings(sg_context *c, float *prep_ms, float *far_ms, float *near_ms
 
 int sg_last_tile_stats(sg_context *c, long long *visits, long long *near_visits)
 {
+    return sg_last_precision_stats(c, visits, near_visits, nullptr, nullptr);
+}
+
+int sg_last_precision_stats(sg_context *c, long long *visits, long long *near_visits,
+                            long long *wide_visits, long long *fp64_pairs)
+{
     if (!c) return SG_ERR_ARG;
     if (!c->launched) return SG_ERR_STATE;
-    unsigned long long h[2] = {0, 0};
+    unsigned long long h[4] = {0, 0, 0, 0};
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(h, c->stats, sizeof(h), hipMemcpyDeviceToHost));
     if (visits) *visits = (long long)h[0];
     if (near_visits) *near_visits = (long long)h[1];
+    if (wide_visits) *wide_visits = (long long)h[2];
+    if (fp64_pairs) *fp64_pairs = (long long)h[3];
     return SG_OK;
 }
 
@@ -891,7 +904,7 @@ int sg_vjp_device(sg_context *c, int n, const double *x, const double *y, const 
     if ((rc = grow(&c->nearflag, &c->cap_flag, a.ngroups * ntiles)) != SG_OK) return rc;
     a.nearflag = c->nearflag;
     a.stats = c->stats;
-    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 4, s));
+    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 8, s));
 
     HIP_TRY(hipEventRecord(c->ev_k0, s));
     HIP_TRY(sg::launch_adj_far(precision, a, wg, s));

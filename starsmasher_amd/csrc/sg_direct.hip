@@ -279,7 +279,7 @@ __device__ __forceinline__ void near_chunk_slice(
     const double (&xr)[IPL], const double (&yr)[IPL], const double (&zr)[IPL],
     const double (&Hi)[IPL], const double (&invHi)[IPL], const double (&hi2)[IPL],
     const int (&idi)[IPL], double (&ax)[IPL], double (&ay)[IPL], double (&az)[IPL],
-    double (&ap)[IPL])
+    double (&ap)[IPL], unsigned (&nfp64)[IPL])
 {
     const int lane = threadIdx.x & 63;
     {
@@ -356,6 +356,7 @@ __device__ __forceinline__ void near_chunk_slice(
     }
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
+        nfp64[k] += (unsigned)__builtin_popcountll(own[k]);
         while (own[k]) {
             const int jj = __builtin_ctzll(own[k]);
             own[k] &= own[k] - 1;
@@ -430,7 +431,10 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
     // a wave past the end of its segment (the tail of the last i-block)
     // computes nothing and flags nothing
     const bool active = uniform_i(b.ibase + wv * WI < b.iend) != 0;
-    unsigned nearv = 0;
+    unsigned nearv = 0, widev = 0;   // wave-uniform counts of near and wide chunk visits
+    unsigned nfp64[IPL];             // per lane: pairs evaluated in FP64 (near tail)
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) nfp64[k] = 0;
     int nlist = 0;   // wave-uniform
     unsigned char *flags = b.flags + wv * b.fstride;
     for (int t = b.t0; t < b.t1; t += b.tstep) {
@@ -447,6 +451,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
                               wide_d2) != 0)
                     widem |= 1u << c;
         nearv += __builtin_popcount(mask);
+        if (active) widev += __builtin_popcount(widem & ~mask);
         bool near_tile_now = false;   // a full list: this tile's near chunks run below
         if (!FUSED) {
             if (lane == 0) flags[t - b.fbase] = (unsigned char)mask;
@@ -466,7 +471,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
             for (int c = 0; c < CHUNKS; ++c)
                 if ((mask >> c) & 1u)
                     near_chunk_slice<IPL, NK>(s_near[wv], J, t, c, Ox, Oy, Oz, xr, yr, zr, Hi,
-                                              invHi, hi2, idi, ax, ay, az, ap);
+                                              invHi, hi2, idi, ax, ay, az, ap, nfp64);
         }
         if (WG == 64 && mask == (1u << CHUNKS) - 1) continue;
         for (int s = 0; s < TJ / TS; ++s) {
@@ -504,7 +509,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
             for (int c = 0; c < CHUNKS; ++c)
                 if ((e >> c) & 1u)
                     near_chunk_slice<IPL, NK>(s_near[wv], J, t, c, Ox, Oy, Oz, xr, yr, zr, Hi,
-                                              invHi, hi2, idi, ax, ay, az, ap);
+                                              invHi, hi2, idi, ax, ay, az, ap, nfp64);
         }
     }
 
@@ -516,9 +521,19 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         b.part[2 * b.pstride + ii[k]] = az[k];
         b.part[3 * b.pstride + ii[k]] = ap[k];
     }
-    if (stats && lane == 0) {
-        atomicAdd(&stats[0], (unsigned long long)((b.t1 - b.t0 + b.tstep - 1) / b.tstep) * CHUNKS);
-        atomicAdd(&stats[1], (unsigned long long)nearv);
+    if (stats) {
+        // i past the end of the set (copies of the last one) are not counted
+        unsigned f = 0;
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) f += valid[k] ? nfp64[k] : 0u;
+        for (int o = 32; o > 0; o >>= 1) f += __shfl_xor(f, o, 64);
+        if (lane == 0) {
+            atomicAdd(&stats[0],
+                      (unsigned long long)((b.t1 - b.t0 + b.tstep - 1) / b.tstep) * CHUNKS);
+            atomicAdd(&stats[1], (unsigned long long)nearv);
+            atomicAdd(&stats[2], (unsigned long long)widev);
+            atomicAdd(&stats[3], (unsigned long long)f);
+        }
     }
 }
 

@@ -19,6 +19,7 @@ is raised instead.  :class:`Gravity` wraps the device-resident sg_* API
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -116,10 +117,41 @@ def bonet_psi(x, y, z, am, hp, u, nkernel=0, ngrav_lower=1, ngrav_upper=None):
     return out
 
 
+def hip_runtimes() -> list:
+    """The distinct HIP runtime libraries (libamdhip64) mapped into this
+    process, by real path."""
+    paths = set()
+    with open("/proc/self/maps") as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 6 and "libamdhip64.so" in parts[-1]:
+                paths.add(os.path.realpath(parts[-1]))
+    return sorted(paths)
+
+
+def _check_shared_runtime(name):
+    """Device pointers and streams of torch are only meaningful to the library
+    when both use one HIP runtime.  torch wheels bundle their own
+    libamdhip64, and a pointer or stream of one runtime is not valid in the
+    other, so a second runtime in the process makes torch tensors unusable
+    here (allocate through starsmasher_amd.hipmem instead)."""
+    _lib.load()
+    rts = hip_runtimes()
+    if len(rts) != 1:
+        raise TypeError(f"{name}: torch tensors cannot be passed: this process maps "
+                        f"{len(rts)} HIP runtimes ({', '.join(rts)}), and device pointers "
+                        "and streams of one are invalid in another; allocate with "
+                        "starsmasher_amd.hipmem.DeviceArray")
+
+
+def _is_torch(t) -> bool:
+    return type(t).__module__.startswith("torch")
+
+
 def _dev_ptr(t, name, n=None):
     """Device pointer of a :class:`starsmasher_amd.hipmem.DeviceArray`, a raw
     integer address, or a torch tensor (float64, contiguous, on a GPU; only
-    valid when torch and the library share one HIP runtime)."""
+    accepted when torch and the library share one HIP runtime)."""
     if isinstance(t, int):
         return t
     if hasattr(t, "ptr") and hasattr(t, "size"):
@@ -132,6 +164,7 @@ def _dev_ptr(t, name, n=None):
         raise TypeError(f"{name}: pass an integer device address or a torch tensor") from e
     if not isinstance(t, torch.Tensor):
         raise TypeError(f"{name} must be a torch tensor on the GPU or a device address")
+    _check_shared_runtime(name)
     if t.device.type != "cuda" or t.dtype != torch.float64 or not t.is_contiguous():
         raise TypeError(f"{name} must be a contiguous float64 tensor on a GPU")
     if n is not None and t.numel() < n:
@@ -169,7 +202,7 @@ class Gravity:
         self.close()
 
     # ------------------------------------------------------------ config
-    _stream = None
+    _stream = 0          # 0: the context's own stream
 
     def set_stream(self, stream):
         """Enqueue on `stream` (a hipStream_t address; 0/None = own stream)."""
@@ -254,22 +287,32 @@ class Gravity:
                                                   (wz, "wz"), (u, "u"), (gx, "gx"), (gy, "gy"),
                                                   (gz, "gz"), (gm, "gm"), (gH, "gH"))]
         vp = [C.c_void_p(p) for p in ptrs]
+        self._stream_for(x)
         st = self._L.sg_vjp_device(self._h, n, *vp[:5], int(nkernel), int(precision), *vp[5:])
         check(st, "sg_vjp_device")
 
+    def _stream_for(self, t):
+        """Enqueue on the caller's current torch stream when `t` is a torch
+        tensor (ordered after the work that produced it), else on the
+        context's own stream, so a torch call's stream does not linger into
+        later calls on device arrays."""
+        if _is_torch(t):
+            _check_shared_runtime("stream")
+            import torch
+            self.set_stream(torch.cuda.current_stream(t.device).cuda_stream)
+        else:
+            self.set_stream(None)
+
     def compute_device(self, n, x, y, z, m, range2, ax, ay, az, pot, nkernel=0,
                        precision=SG_PREC_FP32C, ibeg=0, ni=None):
-        """Enqueue a device-resident evaluation (torch tensors or addresses)."""
+        """Enqueue a device-resident evaluation (device arrays, addresses, or
+        torch tensors when torch shares the library's HIP runtime)."""
         n = int(n)
         ni = n - ibeg if ni is None else int(ni)
-        if type(x).__module__.startswith("torch"):
-            # run on the caller's current torch stream, ordered after the
-            # work that produced the inputs
-            import torch
-            self.set_stream(torch.cuda.current_stream(x.device).cuda_stream)
         ins = [_dev_ptr(a, nm, n) for a, nm in ((x, "x"), (y, "y"), (z, "z"), (m, "m"),
                                                  (range2, "range2"))]
         outs = [_dev_ptr(a, nm, ni) for a, nm in ((ax, "ax"), (ay, "ay"), (az, "az"), (pot, "pot"))]
+        self._stream_for(x)
         st = self._L.sg_compute_device(self._h, n, int(ibeg), ni, *[C.c_void_p(p) for p in ins],
                                        int(nkernel), int(precision),
                                        *[C.c_void_p(p) for p in outs])
@@ -296,6 +339,7 @@ class Gravity:
                                                     (range2, "range2"))]
         outs = [_dev_ptr(a, nm, ntot) for a, nm in ((ax, "ax"), (ay, "ay"), (az, "az"),
                                                      (pot, "pot"))]
+        self._stream_for(x)
         st = self._L.sg_compute_batch_device(
             self._h, off.size - 1, off.ctypes.data_as(C.POINTER(C.c_longlong)),
             *[C.c_void_p(p) for p in ins], int(nkernel), int(precision), int(bool(use_graph)),
@@ -326,6 +370,7 @@ class Gravity:
         of output.f:232-252 without copying the potentials to the host."""
         n = int(n)
         mp, pp = _dev_ptr(m, "m", n), _dev_ptr(pot, "pot", n)
+        self._stream_for(m)
         e = C.c_double()
         check(self._L.sg_potential_energy(self._h, n, mp, pp, C.byref(e)), "sg_potential_energy")
         return e.value
@@ -345,3 +390,11 @@ class Gravity:
         v, nv = C.c_longlong(), C.c_longlong()
         check(self._L.sg_last_tile_stats(self._h, C.byref(v), C.byref(nv)), "sg_last_tile_stats")
         return v.value, nv.value
+
+    def last_precision_stats(self):
+        """(chunk visits, near chunk visits, wide chunk visits, FP64 pairs) of
+        the last evaluation (sg_last_precision_stats)."""
+        v = [C.c_longlong() for _ in range(4)]
+        check(self._L.sg_last_precision_stats(self._h, *[C.byref(x) for x in v]),
+              "sg_last_precision_stats")
+        return tuple(x.value for x in v)

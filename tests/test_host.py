@@ -37,3 +37,32 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_torch_tensors_refused_across_hip_runtimes():
+    """torch bundles its own HIP runtime.  When the library is loaded first
+    it binds ROCm's runtime, and importing torch then maps a second one: a
+    torch tensor's pointer (and torch's stream) would reach a runtime that
+    does not know them, so the device API refuses torch tensors with
+    TypeError instead of enqueueing work on them.  Run in a fresh process so
+    the load order is known."""
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from starsmasher_amd import _lib, gravity\n"
+        "_lib.load()\n"
+        "import torch\n"
+        "rts = gravity.hip_runtimes()\n"
+        "assert len(rts) == 2, rts\n"
+        "try:\n"
+        "    gravity._dev_ptr(torch.zeros(8, dtype=torch.float64), 'x', 8)\n"
+        "except TypeError as e:\n"
+        "    assert 'HIP runtimes' in str(e), e\n"
+        "else:\n"
+        "    raise SystemExit('torch tensor accepted across runtimes')\n"
+        "assert gravity._dev_ptr(12345, 'x') == 12345\n"
+        "print('refused')\n" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "refused"
