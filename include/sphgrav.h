@@ -186,7 +186,8 @@ int sg_last_tile_stats(sg_context *ctx, long long *visits, long long *near_visit
  * in FP64 and rounded once (chunks close to the wave relative to its spread
  * about the workgroup origin), and fp64_pairs = (i, j) pairs evaluated
  * entirely in FP64 (those that may be softened, r < max(H_i, H_j), the
- * self pair included).  A chunk visit covers 64 j for each i of a wave
+ * self pair included; pairs with the massless padding that fills the last
+ * j-tile are evaluated too but not counted).  A chunk visit covers 64 j for each i of a wave
  * (64 per lane and i-particle per lane).  Zero for the FP64 pass, whose
  * pairs are all FP64.  Any pointer may be NULL. */
 int sg_last_precision_stats(sg_context *ctx, long long *visits, long long *near_visits,

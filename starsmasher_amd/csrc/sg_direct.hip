@@ -282,11 +282,13 @@ __device__ __forceinline__ void near_chunk_slice(
     double (&ap)[IPL], unsigned (&nfp64)[IPL])
 {
     const int lane = threadIdx.x & 63;
+    uint64_t real;   // wave-uniform: j that are particles (not massless padding)
     {
         const int64_t j = (int64_t)t * TJ + c * TC + lane;
         const double rx = J.x[j] - Ox, ry = J.y[j] - Oy, rz = J.z[j] - Oz, mj = J.m[j];
         const double hs = J.Hs[j];
         const float mh[2] = {(float)mj, (float)(hs * hs)};
+        real = __ballot(mj > 0.0);
         wave_lds_sync();
         sl.p[lane] = make_double4(rx, ry, rz, __builtin_bit_cast(double, mh));
         sl.hs[lane] = hs;
@@ -356,7 +358,9 @@ __device__ __forceinline__ void near_chunk_slice(
     }
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
-        nfp64[k] += (unsigned)__builtin_popcountll(own[k]);
+        // the counter covers pairs of particles; padding pairs (m_j = 0,
+        // evaluated all the same) are left out
+        nfp64[k] += (unsigned)__builtin_popcountll(own[k] & real);
         while (own[k]) {
             const int jj = __builtin_ctzll(own[k]);
             own[k] &= own[k] - 1;
