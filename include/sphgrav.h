@@ -76,7 +76,8 @@ enum sg_status {
     SG_ERR_HIP = 6,          /* HIP runtime error                           */
     SG_ERR_NO_DEVICE = 7,    /* no HIP device / device index out of range   */
     SG_ERR_STATE = 8,        /* lasthalf without a pending firsthalf         */
-    SG_ERR_CONFIG = 9        /* unsupported tile configuration              */
+    SG_ERR_CONFIG = 9,       /* unsupported tile configuration              */
+    SG_ERR_SIZE = 10         /* n above SG_STATIC_MAX_N (static entry point) */
 };
 
 enum sg_precision {
@@ -272,6 +273,41 @@ int sg_lasthalf(double *ax, double *ay, double *az, double *pot);
 int sg_grav_psi(int n, int n_lower, int my_length,
                 const double *x, const double *y, const double *z,
                 const double *m, const double *range2, int nkernel, double *psi);
+
+/* ------------------------------------------------------------------------ */
+/* Evaluation without device allocation                                      */
+/* ------------------------------------------------------------------------ */
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The evaluation of sg_compute_device -- same
+ * arguments and results as a context at its default configuration (tile
+ * shape 256 x 1, automatic j-split) -- on the CURRENT HIP device, enqueued
+ * on `stream` (a hipStream_t of that device; NULL = the null stream).  It
+ * allocates nothing: every scratch buffer (sorted copies, tile boxes,
+ * partial sums, sort storage, counters) is carved from one file-scope
+ * device array of SG_STATIC_SCRATCH_BYTES per device, held by each device
+ * the library runs on until the process ends.
+ *   - n above SG_STATIC_MAX_N returns SG_ERR_SIZE;
+ *   - every call on a device shares that device's array and leaves its
+ *     contents behind: calls that overlap on one device (different streams
+ *     or host threads) race; calls in one stream are ordered;
+ *   - input validation runs on the device: when `err` (a DEVICE int) is
+ *     given it receives the bit flags SG_VALID_* (0 = inputs valid), readable
+ *     once the stream reaches the end of the call. */
+#define SG_STATIC_MAX_N 32768
+#define SG_STATIC_SCRATCH_BYTES 41591552
+enum sg_validation_flags {
+    SG_VALID_MASS = 1,       /* a mass <= 0                                 */
+    SG_VALID_RANGE2 = 2,     /* a range2 == 0                               */
+    SG_VALID_NONFINITE = 4   /* NaN / Inf in positions, masses or range2    */
+};
+int sg_compute_device_static(int n, int ibeg, int ni,
+                             const double *x, const double *y, const double *z,
+                             const double *m, const double *range2,
+                             int nkernel, int precision,
+                             double *ax, double *ay, double *az, double *pot,
+                             int *err, void *stream);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,13 @@ SG_ERR_HIP = 6
 SG_ERR_NO_DEVICE = 7
 SG_ERR_STATE = 8
 SG_ERR_CONFIG = 9
+SG_ERR_SIZE = 10
+
+# sg_compute_device_static: largest n, and the bit flags it writes to *err
+SG_STATIC_MAX_N = 32768
+SG_VALID_MASS = 1
+SG_VALID_RANGE2 = 2
+SG_VALID_NONFINITE = 4
 
 SG_PREC_FP32C = 0
 SG_PREC_FP64 = 1
@@ -35,7 +42,7 @@ EXPORTS = (
     "sg_set_config", "sg_get_config", "sg_last_timings", "sg_last_tile_stats", "sg_init_device",
     "sg_firsthalf", "sg_lasthalf", "sg_vjp_device", "sg_vjp_host", "sg_compute_psi_device",
     "sg_compute_psi_host", "sg_psi_device", "sg_psi_host", "sg_grav_psi", "sg_potential_energy",
-    "sg_last_precision_stats",
+    "sg_last_precision_stats", "sg_compute_device_static",
 )
 
 
@@ -106,12 +113,15 @@ def load() -> C.CDLL:
     L.sg_vjp_device.argtypes = [_vp, C.c_int] + [_vp] * 5 + [C.c_int, C.c_int] + [_vp] * 9
     L.sg_vjp_host.argtypes = [_vp, C.c_int] + [_dp] * 5 + [C.c_int, C.c_int] + [_dp] * 9
     L.sg_potential_energy.argtypes = [_vp, C.c_int, _vp, _vp, C.POINTER(C.c_double)]
+    L.sg_compute_device_static.argtypes = ([C.c_int, C.c_int, C.c_int] + [_vp] * 5
+                                           + [C.c_int, C.c_int] + [_vp] * 4 + [_vp, _vp])
     for name in ("sg_last_status", "sg_device_count", "sg_create", "sg_destroy", "sg_compute_device",
                  "sg_compute_host", "sg_synchronize", "sg_last_kernel_ms", "sg_set_config",
                  "sg_get_config", "sg_last_timings", "sg_last_tile_stats", "sg_init_device",
                  "sg_firsthalf", "sg_lasthalf", "sg_vjp_device", "sg_vjp_host",
                  "sg_compute_psi_device", "sg_compute_psi_host", "sg_psi_device",
-                 "sg_psi_host", "sg_grav_psi", "sg_potential_energy", "sg_last_precision_stats"):
+                 "sg_psi_host", "sg_grav_psi", "sg_potential_energy", "sg_last_precision_stats",
+                 "sg_compute_device_static"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

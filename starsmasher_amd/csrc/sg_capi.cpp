@@ -195,6 +195,18 @@ int ensure_capacity(sg_context *c, int64_t n, int64_t padded, int64_t ni)
     return SG_OK;
 }
 
+bool env_flag(const char *name)
+{
+    const char *v = std::getenv(name);
+    return v && *v && std::strcmp(v, "0") != 0;
+}
+
+}  // namespace
+
+namespace sg {
+
+void record_hip_error(hipError_t e) { last_hip_error() = e; }
+
 int auto_jsplit(int niblocks, int wg, int ntiles, int64_t ni)
 {
     // Split the j range until the launch holds about 2^18 waves (some 36
@@ -212,13 +224,99 @@ int auto_jsplit(int niblocks, int wg, int ntiles, int64_t ni)
     return (int)std::max<int64_t>(1, std::min(want, cap));
 }
 
-bool env_flag(const char *name)
+EvalPlan plan_eval(int n, int ni, int wg, int ipl, int jsplit_cfg, int precision, bool psi)
 {
-    const char *v = std::getenv(name);
-    return v && *v && std::strcmp(v, "0") != 0;
+    EvalPlan p{};
+    p.padded = ((int64_t)n + TJ - 1) / TJ * TJ;
+    p.ntiles = (int)(p.padded / TJ);
+    // the grad-h term is formed by the near kernel with one i per lane
+    p.wg = wg;
+    p.ipl = psi ? 1 : ipl;
+    p.niblocks = (int)((ni + (int64_t)p.wg * p.ipl - 1) / ((int64_t)p.wg * p.ipl));
+    p.ngroups = (int64_t)p.niblocks * (p.wg / 64);
+    p.jsplit = jsplit_cfg > 0 ? std::min(jsplit_cfg, p.ntiles)
+                              : auto_jsplit(p.niblocks, p.wg, p.ntiles, ni);
+    p.part_elems = (int64_t)p.jsplit * 4 * ni;
+    // per-(wave, tile) near flags, N/64 x N/256 bytes: only the passes with a
+    // separate near kernel (FP64, grad-h) read them; the fused FP32C pass
+    // keeps its near list in LDS
+    const bool flagged = psi || precision != SG_PREC_FP32C;
+    p.flag_elems = flagged ? p.ngroups * p.ntiles : 1;
+    p.psipart_elems = psi ? (int64_t)p.jsplit * ni : 0;
+    return p;
 }
 
-}  // namespace
+hipError_t enqueue_eval(const EvalPlan &p, const EvalBuffers &bufs, int n, int ibeg, int ni,
+                        const double *x, const double *y, const double *z, const double *m,
+                        const double *range2, int nkernel, int precision, double *ax, double *ay,
+                        double *az, double *pot, double *psi, hipStream_t s,
+                        const hipEvent_t *ev)
+{
+#define SG_ENQ(expr)                          \
+    do {                                      \
+        hipError_t e_ = (expr);               \
+        if (e_ != hipSuccess) return e_;      \
+    } while (0)
+    PrepBuffers b = bufs.pb;
+    if (ev) SG_ENQ(hipEventRecord(ev[0], s));
+    // 1-2: validate + bbox, Morton keys of all n
+    SG_ENQ(launch_validate_bbox(x, y, z, m, range2, n, b, s));
+    SG_ENQ(launch_morton(x, y, z, 0, n, b, s));
+    // 3: sort
+    SG_ENQ(sort_pairs(bufs.sort_tmp, bufs.sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
+                      b.vals_out, n, s));
+    // 4-5: gather sorted j arrays, tile metadata
+    SG_ENQ(launch_gather(x, y, z, m, range2, b.vals_out, n, p.padded, b.sx, b.sy, b.sz, b.sm,
+                         b.sH, b.sidx, s));
+    SG_ENQ(launch_tile_meta(b.sx, b.sy, b.sz, b.sH, p.ntiles, b.tiles, b.chunks, s));
+
+    PairArgs a{};
+    a.j = SortedSet{b.sx, b.sy, b.sz, b.sm, b.sH, b.sidx, n, p.padded};
+    if (ibeg == 0 && ni == n) {
+        a.i = a.j;
+    } else {
+        // the i-range gets its own curve order
+        SG_ENQ(launch_morton(x, y, z, ibeg, ni, b, s));
+        SG_ENQ(sort_pairs(bufs.sort_tmp, bufs.sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
+                          b.vals_out, ni, s));
+        SG_ENQ(launch_gather(x, y, z, m, range2, b.vals_out, ni, ni, b.ix, b.iy, b.iz, b.im,
+                             b.iH, b.iidx, s));
+        a.i = SortedSet{b.ix, b.iy, b.iz, b.im, b.iH, b.iidx, ni, ni};
+    }
+    a.tiles = b.tiles;
+    a.chunks = b.chunks;
+    a.ni = ni;
+    a.ibeg = ibeg;
+    a.nkernel = nkernel;
+    a.ntiles = p.ntiles;
+    a.ipl = p.ipl;
+    a.niblocks = p.niblocks;
+    a.ngroups = p.ngroups;
+    a.jsplit = p.jsplit;
+    a.ax = ax;
+    a.ay = ay;
+    a.az = az;
+    a.pot = pot;
+    a.partial = bufs.partial;
+    a.nearflag = bufs.nearflag;
+    a.stats = bufs.stats;
+    if (psi) {
+        a.psi = psi;
+        a.psipart = bufs.psipart;
+    }
+    SG_ENQ(launch_fill_zero_int((int *)bufs.stats, 8, s));
+
+    if (ev) SG_ENQ(hipEventRecord(ev[1], s));
+    SG_ENQ(launch_pair_far(precision, a, p.wg, s));
+    if (ev) SG_ENQ(hipEventRecord(ev[2], s));
+    SG_ENQ(launch_pair_near(precision, a, s));
+    SG_ENQ(launch_reduce_partials(a, s));
+    if (ev) SG_ENQ(hipEventRecord(ev[3], s));
+#undef SG_ENQ
+    return hipSuccess;
+}
+
+}  // namespace sg
 
 // ============================================================ sg_* API
 
@@ -237,6 +335,7 @@ const char *sg_status_string(int status)
     case SG_ERR_NO_DEVICE: return "no usable HIP device";
     case SG_ERR_STATE: return "lasthalf called without a pending firsthalf";
     case SG_ERR_CONFIG: return "unsupported tile configuration";
+    case SG_ERR_SIZE: return "n exceeds SG_STATIC_MAX_N, the limit of sg_compute_device_static";
     default: return "unknown status";
     }
 }
@@ -349,79 +448,19 @@ int compute_device_impl(sg_context *c, int n, int ibeg, int ni, const double *x,
     c->launched = false;
     if (ni == 0) return SG_OK;
 
-    const int64_t padded = ((int64_t)(n) + TJ - 1) / TJ * TJ;
-    int rc = ensure_capacity(c, n, padded, ni);
+    const sg::EvalPlan p = sg::plan_eval(n, ni, c->wg[precision], c->ipl[precision],
+                                         c->jsplit[precision], precision, psi != nullptr);
+    if (psi && p.psipart_elems > INT32_MAX / 2) return SG_ERR_ARG;
+    int rc = ensure_capacity(c, n, p.padded, ni);
     if (rc != SG_OK) return rc;
-    hipStream_t s = c->stream;
-    sg::PrepBuffers &b = c->pb;
-    HIP_TRY(hipEventRecord(c->ev_p0, s));
-
-    // 1-2: validate + bbox, Morton keys of all n
-    HIP_TRY(sg::launch_validate_bbox(x, y, z, m, range2, n, b, s));
-    HIP_TRY(sg::launch_morton(x, y, z, 0, n, b, s));
-    // 3: sort
-    HIP_TRY(sg::sort_pairs(c->sort_tmp, c->sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
-                           b.vals_out, n, s));
-    // 4-5: gather sorted j arrays, tile metadata
-    HIP_TRY(sg::launch_gather(x, y, z, m, range2, b.vals_out, n, padded, b.sx, b.sy, b.sz, b.sm,
-                              b.sH, b.sidx, s));
-    const int ntiles = (int)(padded / TJ);
-    HIP_TRY(sg::launch_tile_meta(b.sx, b.sy, b.sz, b.sH, ntiles, b.tiles, b.chunks, s));
-
-    sg::PairArgs a{};
-    a.j = sg::SortedSet{b.sx, b.sy, b.sz, b.sm, b.sH, b.sidx, n, padded};
-    if (ibeg == 0 && ni == n) {
-        a.i = a.j;
-    } else {
-        // the i-range gets its own curve order
-        HIP_TRY(sg::launch_morton(x, y, z, ibeg, ni, b, s));
-        HIP_TRY(sg::sort_pairs(c->sort_tmp, c->sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
-                               b.vals_out, ni, s));
-        HIP_TRY(sg::launch_gather(x, y, z, m, range2, b.vals_out, ni, ni, b.ix, b.iy, b.iz, b.im,
-                                  b.iH, b.iidx, s));
-        a.i = sg::SortedSet{b.ix, b.iy, b.iz, b.im, b.iH, b.iidx, ni, ni};
-    }
-    a.tiles = b.tiles;
-    a.chunks = b.chunks;
-    a.ni = ni;
-    a.ibeg = ibeg;
-    a.nkernel = nkernel;
-    a.ntiles = ntiles;
-    // the grad-h term is formed by the near kernel with one i per lane
-    const int wg = c->wg[precision], ipl = psi ? 1 : c->ipl[precision];
-    a.ipl = ipl;
-    a.niblocks = (int)((ni + (int64_t)wg * ipl - 1) / ((int64_t)wg * ipl));
-    a.ngroups = (int64_t)a.niblocks * (wg / 64);
-    a.jsplit = c->jsplit[precision] > 0 ? std::min(c->jsplit[precision], ntiles)
-                                        : auto_jsplit(a.niblocks, wg, ntiles, ni);
-    a.ax = ax;
-    a.ay = ay;
-    a.az = az;
-    a.pot = pot;
-    if ((rc = grow(&c->partial, &c->cap_part, (int64_t)a.jsplit * 4 * ni)) != SG_OK) return rc;
-    a.partial = c->partial;
-    // per-(wave, tile) near flags, N/64 x N/256 bytes: only the passes with a
-    // separate near kernel (FP64, grad-h) read them; the fused FP32C pass
-    // keeps its near list in LDS
-    const bool flagged = psi || precision != SG_PREC_FP32C;
-    if ((rc = grow(&c->nearflag, &c->cap_flag, flagged ? a.ngroups * ntiles : 1)) != SG_OK)
-        return rc;
-    a.nearflag = c->nearflag;
-    a.stats = c->stats;
-    if (psi) {
-        if ((int64_t)a.jsplit * ni > INT32_MAX / 2) return SG_ERR_ARG;
-        if ((rc = grow(&c->psipart, &c->cap_psipart, (int64_t)a.jsplit * ni)) != SG_OK) return rc;
-        a.psi = psi;
-        a.psipart = c->psipart;
-    }
-    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 8, s));
-
-    HIP_TRY(hipEventRecord(c->ev_k0, s));
-    HIP_TRY(sg::launch_pair_far(precision, a, wg, s));
-    HIP_TRY(hipEventRecord(c->ev_kf, s));
-    HIP_TRY(sg::launch_pair_near(precision, a, s));
-    HIP_TRY(sg::launch_reduce_partials(a, s));
-    HIP_TRY(hipEventRecord(c->ev_k1, s));
+    if ((rc = grow(&c->partial, &c->cap_part, p.part_elems)) != SG_OK) return rc;
+    if ((rc = grow(&c->nearflag, &c->cap_flag, p.flag_elems)) != SG_OK) return rc;
+    if (psi && (rc = grow(&c->psipart, &c->cap_psipart, p.psipart_elems)) != SG_OK) return rc;
+    const sg::EvalBuffers bufs{c->pb,       c->sort_tmp, c->sort_tmp_bytes, c->partial,
+                               c->nearflag, c->stats,    c->psipart};
+    const hipEvent_t ev[4] = {c->ev_p0, c->ev_k0, c->ev_kf, c->ev_k1};
+    HIP_TRY(sg::enqueue_eval(p, bufs, n, ibeg, ni, x, y, z, m, range2, nkernel, precision, ax, ay,
+                             az, pot, psi, c->stream, ev));
     c->launched = true;
     return SG_OK;
 }
@@ -893,7 +932,7 @@ int sg_vjp_device(sg_context *c, int n, const double *x, const double *y, const 
     const int wg = sg::adj_config_supported(c->wg[precision]) ? c->wg[precision] : 256;
     a.niblocks = (int)((n + (int64_t)wg - 1) / wg);
     a.ngroups = (int64_t)a.niblocks * (wg / 64);
-    a.jsplit = auto_jsplit(a.niblocks, wg, ntiles, n);
+    a.jsplit = sg::auto_jsplit(a.niblocks, wg, ntiles, n);
     a.gx = gx;
     a.gy = gy;
     a.gz = gz;

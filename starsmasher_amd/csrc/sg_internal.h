@@ -187,4 +187,43 @@ hipError_t sort_pairs_temp_bytes(int n, size_t *bytes);
 hipError_t sort_pairs(void *tmp, size_t tmp_bytes, const uint64_t *kin, uint64_t *kout,
                       const int *vin, int *vout, int n, hipStream_t s);
 
+// ------------------------------------------------- one evaluation (sg_capi.cpp)
+// Sizes of one evaluation of n particles with ni i-particles at a tile
+// shape; the context path (sg_compute_device) and the static-scratch path
+// (sg_compute_device_static) size their device buffers from it, so both
+// launch the same kernels with the same j-partitions.
+struct EvalPlan {
+    int64_t padded;        // sorted j slots (multiple of TJ)
+    int ntiles;
+    int wg, ipl, jsplit, niblocks;
+    int64_t ngroups;       // waves of the far kernel
+    int64_t part_elems;    // doubles of j-partition partial sums: jsplit * 4 * ni
+    int64_t flag_elems;    // near-flag bytes (FP64 and grad-h passes; else 1)
+    int64_t psipart_elems; // grad-h partial sums: jsplit * ni (with psi; else 0)
+};
+int auto_jsplit(int niblocks, int wg, int ntiles, int64_t ni);
+// the HIP error that sg_status_string(SG_ERR_HIP) describes
+void record_hip_error(hipError_t e);
+EvalPlan plan_eval(int n, int ni, int wg, int ipl, int jsplit_cfg, int precision, bool psi);
+
+// The device memory one evaluation works in.
+struct EvalBuffers {
+    PrepBuffers pb;
+    void *sort_tmp;
+    size_t sort_tmp_bytes;
+    double *partial;
+    unsigned char *nearflag;
+    unsigned long long *stats;   // [4]
+    double *psipart;             // with psi only
+};
+
+// Enqueues validation, sort, gather, tile boxes, the pair kernels and the
+// reduction on s.  ev (may be null) = events recorded before the preparation,
+// before the far kernel, after it and at the end.
+hipError_t enqueue_eval(const EvalPlan &p, const EvalBuffers &b, int n, int ibeg, int ni,
+                        const double *x, const double *y, const double *z, const double *m,
+                        const double *range2, int nkernel, int precision, double *ax, double *ay,
+                        double *az, double *pot, double *psi, hipStream_t s,
+                        const hipEvent_t *ev);
+
 }  // namespace sg

@@ -44,7 +44,7 @@ def test_exported_symbols_in_dynamic_table():
 
 
 def test_status_strings():
-    for st in range(0, 10):
+    for st in range(0, 11):
         assert isinstance(_lib.status_string(st), str) and _lib.status_string(st)
 
 
