@@ -100,8 +100,11 @@ def host_info():
                 models.add(v)
             elif k in ("cpu family", "model", "stepping", "cpu MHz", "cache size") and k not in keys:
                 keys[k] = v
+        # the numeric "model" field is reported as cpu_model_number, so that
+        # it does not overwrite the model name
+        info.update({f"cpu_{k.replace(' ', '_').lower()}" + ("_number" if k == "model" else ""): v
+                     for k, v in keys.items()})
         info["cpu_model"] = "; ".join(sorted(models))
-        info.update({f"cpu_{k.replace(' ', '_').lower()}": v for k, v in keys.items()})
     except OSError:
         pass
     return info
