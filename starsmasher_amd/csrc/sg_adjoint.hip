@@ -239,7 +239,7 @@ __device__ __forceinline__ void adj_chunk_fp64(const double4 *s_p, const double4
         const double4 w = s_w[jj];
         const double dx = p.x - xr, dy = p.y - yr, dz = p.z - zr;
         const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double ri = rsqrt(r2);
+        const double ri = rsqrt_pos64(r2);   // far pairs: r^2 >= max(H_k^2, H_l^2) > 0
         const double ri2 = ri * ri;
         const double ri3 = ri2 * ri;
         const double wkdr = fma(wkx, dx, fma(wky, dy, wkz * dz));

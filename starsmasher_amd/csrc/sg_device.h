@@ -144,6 +144,18 @@ __device__ __forceinline__ double rsqrt64(double x)
     return fma(0.5 * y, e, y);
 }
 
+// 1/sqrt(x) in FP64 for finite x > 0 in the normal range (every Newtonian
+// pair: r^2 >= max(H_i^2, H_j^2) > 0): the hardware estimate v_rsq_f64 and
+// one correction y + y e (1/2 + 3/8 e), e = 1 - x y^2 -- the steps of the
+// library rsqrt(double) in the same operation order, without its fix-up of
+// zero, infinite and denormal inputs (three VALU per pair).
+__device__ __forceinline__ double rsqrt_pos64(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma((-x) * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 // Orders the LDS writes of one wave before its reads (and the reverse).
 __device__ __forceinline__ void wave_lds_sync()
 {
