@@ -35,6 +35,17 @@ __device__ __forceinline__ float wave_max(float v)
     return uniform_f(v);
 }
 
+// Sum over the wave of a per-lane count, from ballots of its bits (no
+// cross-lane moves, so no lane-address registers stay live until the end of
+// a kernel for it).
+__device__ __forceinline__ unsigned long long wave_sum_count(unsigned v)
+{
+    unsigned long long s = 0;
+    for (int bit = 0; bit < 32; ++bit)
+        s += (unsigned long long)__builtin_popcountll(__ballot((v >> bit) & 1u)) << bit;
+    return s;
+}
+
 // Squared distance between a wave's i-box [bl, bh] and tile tm's box, both
 // relative to the FP64 origin O.
 __device__ __forceinline__ float box_dist2(const TileMeta &tm, double Ox, double Oy, double Oz,

@@ -253,7 +253,7 @@ __device__ __forceinline__ void near_i_state(const SortedSet &I, const int64_t (
 
 // Per-wave LDS of the fused FP32C far kernel: the wave's list of deferred
 // near tiles and the slice a near chunk is staged in.
-constexpr int NLIST = 128;   // listed near tiles per wave (later ones are found again)
+constexpr int NLIST = 128;   // deferred near tiles per wave (more run at once)
 struct NearSlice {
     double4 p[TC];           // x, y, z relative to the origin; w: {FP32 m, FP32 H^2}
     double hs[TC];           // signed H (> 0: gas)
@@ -379,9 +379,8 @@ __device__ __forceinline__ void near_chunk_slice(
 // FUSED (the forward FP32C pass): the chunks a wave cannot classify as far
 // are listed and run by the wave alone after its far loop (in its own LDS
 // slice, no block barrier in between, so one wave's near work does not hold
-// the others at the stage barriers); past a full list the wave notes the
-// first unlisted near tile and classifies the tiles from there again after
-// the loop.  Not FUSED (the grad-h pass): they are flagged in b.flags for the
+// the others at the stage barriers); a list that fills up runs the rest at
+// once.  Not FUSED (the grad-h pass): they are flagged in b.flags for the
 // near kernel.
 template <int WG, int IPL, int NK, bool FUSED>
 __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const SortedSet &J,
@@ -440,8 +439,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
     unsigned nfp64[IPL];             // per lane: pairs evaluated in FP64 (near tail)
 #pragma unroll
     for (int k = 0; k < IPL; ++k) nfp64[k] = 0;
-    int nlist = 0;     // wave-uniform
-    int t_over = -1;   // wave-uniform: first near tile that found the list full
+    int nlist = 0;   // wave-uniform
     unsigned char *flags = b.flags + wv * b.fstride;
     for (int t = b.t0; t < b.t1; t += b.tstep) {
         const TileMeta &tm = tiles[t];
@@ -458,6 +456,7 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
                     widem |= 1u << c;
         nearv += __builtin_popcount(mask);
         if (active) widev += __builtin_popcount(widem & ~mask);
+        bool near_tile_now = false;   // a full list: this tile's near chunks run below
         if (!FUSED) {
             if (lane == 0) flags[t - b.fbase] = (unsigned char)mask;
         } else if (mask) {
@@ -465,9 +464,18 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
                 if (lane == 0)
                     s_near[wv].list[nlist] = ((uint32_t)((t - b.t0) / b.tstep) << 4) | mask;
                 ++nlist;
-            } else if (t_over < 0) {
-                t_over = t;
+            } else {
+                near_tile_now = true;
             }
+        }
+        if (FUSED && near_tile_now) {
+            double Hi[IPL], invHi[IPL], hi2[IPL];
+            int idi[IPL];
+            near_i_state<IPL>(I, ii, Hi, invHi, hi2, idi);
+            for (int c = 0; c < CHUNKS; ++c)
+                if ((mask >> c) & 1u)
+                    near_chunk_slice<IPL, NK>(s_near[wv], J, t, c, Ox, Oy, Oz, xr, yr, zr, Hi,
+                                              invHi, hi2, idi, ax, ay, az, ap, nfp64);
         }
         if (WG == 64 && mask == (1u << CHUNKS) - 1) continue;
         for (int s = 0; s < TJ / TS; ++s) {
@@ -493,34 +501,17 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
             }
         }
     }
-    // The wave's deferred near chunks: the listed tiles, then, when the list
-    // filled up, every near tile from the first one that found it full,
-    // classified again (near_mask gives the same chunks as in the loop).
-    // Keeping all near work after the tile loop leaves the loop the
-    // registers of the far chunks alone.
-    if (FUSED && (nlist > 0 || t_over >= 0)) {
+    // the wave's deferred near chunks
+    if (FUSED && nlist > 0) {
         double Hi[IPL], invHi[IPL], hi2[IPL];
         int idi[IPL];
         near_i_state<IPL>(I, ii, Hi, invHi, hi2, idi);
         wave_lds_sync();
-        int q = 0, tr = t_over;
-        for (;;) {
-            int t;
-            unsigned m;
-            if (q < nlist) {
-                const uint32_t e = s_near[wv].list[q++];
-                t = b.t0 + (int)(e >> 4) * b.tstep;
-                m = e & 15u;
-            } else {
-                if (tr < 0 || tr >= b.t1) break;
-                t = tr;
-                tr += b.tstep;
-                const TileMeta &tm = tiles[t];
-                m = near_mask<IPL>(chunks, t, tm, box_dist2(tm, Ox, Oy, Oz, bl, bh), Ox, Oy, Oz,
-                                   bl, bh, hmax, xh, yh, zh, hf);
-            }
+        for (int q = 0; q < nlist; ++q) {
+            const uint32_t e = s_near[wv].list[q];
+            const int t = b.t0 + (int)(e >> 4) * b.tstep;
             for (int c = 0; c < CHUNKS; ++c)
-                if ((m >> c) & 1u)
+                if ((e >> c) & 1u)
                     near_chunk_slice<IPL, NK>(s_near[wv], J, t, c, Ox, Oy, Oz, xr, yr, zr, Hi,
                                               invHi, hi2, idi, ax, ay, az, ap, nfp64);
         }
@@ -539,13 +530,13 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         unsigned f = 0;
 #pragma unroll
         for (int k = 0; k < IPL; ++k) f += valid[k] ? nfp64[k] : 0u;
-        for (int o = 32; o > 0; o >>= 1) f += __shfl_xor(f, o, 64);
+        const unsigned long long fw = wave_sum_count(f);
         if (lane == 0) {
             atomicAdd(&stats[0],
                       (unsigned long long)((b.t1 - b.t0 + b.tstep - 1) / b.tstep) * CHUNKS);
             atomicAdd(&stats[1], (unsigned long long)nearv);
             atomicAdd(&stats[2], (unsigned long long)widev);
-            atomicAdd(&stats[3], (unsigned long long)f);
+            atomicAdd(&stats[3], fw);
         }
     }
 }
