@@ -263,8 +263,9 @@ hipError_t enqueue_eval(const EvalPlan &p, const EvalBuffers &bufs, int n, int i
     SG_ENQ(launch_validate_bbox(x, y, z, m, range2, n, b, s));
     SG_ENQ(launch_morton(x, y, z, 0, n, b, s));
     // 3: sort
-    SG_ENQ(sort_pairs(bufs.sort_tmp, bufs.sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
-                      b.vals_out, n, s));
+    auto sort = bufs.merge_sort ? merge_sort_pairs : sort_pairs;
+    SG_ENQ(sort(bufs.sort_tmp, bufs.sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in, b.vals_out,
+                n, s));
     // 4-5: gather sorted j arrays, tile metadata
     SG_ENQ(launch_gather(x, y, z, m, range2, b.vals_out, n, p.padded, b.sx, b.sy, b.sz, b.sm,
                          b.sH, b.sidx, s));
@@ -277,8 +278,8 @@ hipError_t enqueue_eval(const EvalPlan &p, const EvalBuffers &bufs, int n, int i
     } else {
         // the i-range gets its own curve order
         SG_ENQ(launch_morton(x, y, z, ibeg, ni, b, s));
-        SG_ENQ(sort_pairs(bufs.sort_tmp, bufs.sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
-                          b.vals_out, ni, s));
+        SG_ENQ(sort(bufs.sort_tmp, bufs.sort_tmp_bytes, b.keys_in, b.keys_out, b.vals_in,
+                    b.vals_out, ni, s));
         SG_ENQ(launch_gather(x, y, z, m, range2, b.vals_out, ni, ni, b.ix, b.iy, b.iz, b.im,
                              b.iH, b.iidx, s));
         a.i = SortedSet{b.ix, b.iy, b.iz, b.im, b.iH, b.iidx, ni, ni};
@@ -457,7 +458,7 @@ int compute_device_impl(sg_context *c, int n, int ibeg, int ni, const double *x,
     if ((rc = grow(&c->nearflag, &c->cap_flag, p.flag_elems)) != SG_OK) return rc;
     if (psi && (rc = grow(&c->psipart, &c->cap_psipart, p.psipart_elems)) != SG_OK) return rc;
     const sg::EvalBuffers bufs{c->pb,       c->sort_tmp, c->sort_tmp_bytes, c->partial,
-                               c->nearflag, c->stats,    c->psipart};
+                               c->nearflag, c->stats,    c->psipart,        false};
     const hipEvent_t ev[4] = {c->ev_p0, c->ev_k0, c->ev_kf, c->ev_k1};
     HIP_TRY(sg::enqueue_eval(p, bufs, n, ibeg, ni, x, y, z, m, range2, nkernel, precision, ax, ay,
                              az, pot, psi, c->stream, ev));

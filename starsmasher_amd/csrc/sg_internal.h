@@ -186,6 +186,11 @@ hipError_t launch_gather_batch(const double *x, const double *y, const double *z
 hipError_t sort_pairs_temp_bytes(int n, size_t *bytes);
 hipError_t sort_pairs(void *tmp, size_t tmp_bytes, const uint64_t *kin, uint64_t *kout,
                       const int *vin, int *vout, int n, hipStream_t s);
+// The same stable order by merge sort: kernels only, no memset (for work a
+// caller may capture in a graph).
+hipError_t merge_sort_pairs_temp_bytes(int n, size_t *bytes);
+hipError_t merge_sort_pairs(void *tmp, size_t tmp_bytes, const uint64_t *kin, uint64_t *kout,
+                            const int *vin, int *vout, int n, hipStream_t s);
 
 // ------------------------------------------------- one evaluation (sg_capi.cpp)
 // Sizes of one evaluation of n particles with ni i-particles at a tile
@@ -215,6 +220,7 @@ struct EvalBuffers {
     unsigned char *nearflag;
     unsigned long long *stats;   // [4]
     double *psipart;             // with psi only
+    bool merge_sort;             // sort with merge_sort_pairs (same order, no memset)
 };
 
 // Enqueues validation, sort, gather, tile boxes, the pair kernels and the

@@ -14,7 +14,10 @@
 //     the last call's contents: calls that overlap on one device (other
 //     streams, other host threads) race; calls in one stream are ordered.
 //   * what must start at zero (validation flags, counters) is cleared by
-//     fill kernels of the call itself, never by hipMemsetAsync.
+//     fill kernels of the call itself, and the Morton sort is rocPRIM's
+//     merge sort (the same stable order as the context path's radix sort,
+//     whose counters are cleared with hipMemsetAsync): the call issues
+//     kernels only, so a caller can capture it in a graph.
 #include "sphgrav.h"
 #include "sg_internal.h"
 
@@ -33,7 +36,7 @@ constexpr int64_t MAXPAD = (MAXN + TJ - 1) / TJ * TJ;
 constexpr int WG = 256, IPL = 1;
 constexpr int64_t MAX_JSPLIT = std::min<int64_t>(64, std::max<int64_t>(1, MAXPAD / TJ / 4));
 constexpr int NRED_BLOCKS = 1024;
-constexpr size_t SORT_RESERVE = (size_t)4 << 20;   // rocPRIM radix-sort temporary storage
+constexpr size_t SORT_RESERVE = (size_t)4 << 20;   // rocPRIM merge-sort temporary storage
 
 constexpr size_t al(size_t b) { return (b + 255) / 256 * 256; }
 
@@ -139,7 +142,7 @@ extern "C" int sg_compute_device_static(int n, int ibeg, int ni, const double *x
 
     const sg::EvalPlan p = sg::plan_eval(n, ni, WG, IPL, 0, precision, false);
     size_t sort_bytes = 0;
-    hipError_t e = sg::sort_pairs_temp_bytes(n, &sort_bytes);
+    hipError_t e = sg::merge_sort_pairs_temp_bytes(n, &sort_bytes);
     if (e != hipSuccess) {
         sg::record_hip_error(e);
         return SG_ERR_HIP;
@@ -175,6 +178,7 @@ extern "C" int sg_compute_device_static(int n, int ibeg, int ni, const double *x
     b.nearflag = base + LAYOUT.nearflag;
     b.stats = at<unsigned long long>(base, LAYOUT.stats);
     b.psipart = nullptr;
+    b.merge_sort = true;   // kernels only: nothing a stream capture would record as a memset
 
     e = sg::enqueue_eval(p, b, n, ibeg, ni, x, y, z, m, range2, nkernel, precision, ax, ay, az,
                          pot, nullptr, (hipStream_t)stream, nullptr);

@@ -7,7 +7,10 @@ kernels with the same j-partitions and reduction order: the outputs must be
 bit-identical, in both precisions, on two calls in a row (the second with a
 smaller set and an i-subrange, so stale contents of the shared array from the
 first call would show).  The oracle checks the first call's FP32C results at
-the usual tolerance (acceleration max 1e-5, potential 1e-6).
+the usual tolerance (acceleration max 1e-5, potential 1e-6).  The entry point
+is also captured in a HIP graph and replayed (the Morton sort is rocPRIM's
+merge sort there, which issues kernels only; it gives the radix sort's order,
+so the results stay bit-identical).
 """
 import ctypes as C
 
@@ -92,3 +95,80 @@ def test_static_reports_validation_flags():
     _static(L, d, 0, 3000, 0, SG_PREC_FP32C, err)
     assert err.download()[0] == _lib.SG_VALID_MASS
     err.free()
+
+
+def _hip():
+    h = runtime()
+    vp = C.c_void_p
+    for name, args in (("hipStreamCreate", [C.POINTER(vp)]), ("hipStreamDestroy", [vp]),
+                       ("hipStreamSynchronize", [vp]),
+                       ("hipStreamBeginCapture", [vp, C.c_int]),
+                       ("hipStreamEndCapture", [vp, C.POINTER(vp)]),
+                       ("hipGraphInstantiate", [C.POINTER(vp), vp, vp, vp, C.c_size_t]),
+                       ("hipGraphLaunch", [vp, vp]), ("hipGraphExecDestroy", [vp]),
+                       ("hipGraphDestroy", [vp])):
+        f = getattr(h, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    return h
+
+
+def _gset(n, seed):
+    d = ics.plummer(n, seed=seed)
+    d["range2"][-2:] *= -1.0
+    return d
+
+
+def _call(L, ins, outs, n, nk, prec, stream):
+    rc = L.sg_compute_device_static(n, 0, n, *[C.c_void_p(a.ptr) for a in ins], nk, prec,
+                                    *[C.c_void_p(o.ptr) for o in outs], None, stream)
+    assert rc == 0, _lib.status_string(rc)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+def test_static_entry_point_in_a_graph(prec):
+    """Captured in a HIP graph (global capture mode, which refuses device
+    allocation while the capture is open) and replayed several times, with
+    eager calls on another set between the replays (they overwrite the
+    shared scratch): every replay reproduces the eager result bit for bit."""
+    L = _lib.load()
+    h = _hip()
+    s = C.c_void_p()
+    assert h.hipStreamCreate(C.byref(s)) == 0
+    n, n2 = 12000, 7000
+    d, d2 = _gset(n, 71), _gset(n2, 72)
+    ins = [DeviceArray.from_host(d[k]) for k in ("x", "y", "z", "m", "range2")]
+    ins2 = [DeviceArray.from_host(d2[k]) for k in ("x", "y", "z", "m", "range2")]
+    eager = [DeviceArray(n) for _ in range(4)]
+    outs = [DeviceArray(n) for _ in range(4)]
+    outs2 = [DeviceArray(n2) for _ in range(4)]
+    # eager reference (also looks up the scratch address before any capture)
+    _call(L, ins, eager, n, 0, prec, s)
+    assert h.hipStreamSynchronize(s) == 0
+    want = [o.download() for o in eager]
+    graph, exe = C.c_void_p(), C.c_void_p()
+    print(f"[graph prec={prec}] capture", flush=True)
+    assert h.hipStreamBeginCapture(s, 0) == 0          # hipStreamCaptureModeGlobal
+    _call(L, ins, outs, n, 0, prec, s)
+    assert h.hipStreamEndCapture(s, C.byref(graph)) == 0
+    assert h.hipGraphInstantiate(C.byref(exe), graph, None, None, 0) == 0
+    try:
+        for rep in range(4):
+            for o in outs:
+                o.upload(np.full(n, np.nan))
+            assert h.hipGraphLaunch(exe, s) == 0
+            assert h.hipStreamSynchronize(s) == 0
+            got = [o.download() for o in outs]
+            print(f"[graph prec={prec}] replay {rep}", flush=True)
+            for a, b in zip(got, want):
+                assert np.array_equal(a, b), rep
+            # eager work between replays: another set through the same scratch
+            _call(L, ins2, outs2, n2, 2, prec, s)
+            assert h.hipStreamSynchronize(s) == 0
+    finally:
+        h.hipGraphExecDestroy(exe)
+        h.hipGraphDestroy(graph)
+        h.hipStreamDestroy(s)
+        for a in ins + ins2 + eager + outs + outs2:
+            a.free()
