@@ -301,7 +301,7 @@ int sg_grav_psi(int n, int n_lower, int my_length,
  *     given it receives the bit flags SG_VALID_* (0 = inputs valid), readable
  *     once the stream reaches the end of the call. */
 #define SG_STATIC_MAX_N 32768
-#define SG_STATIC_SCRATCH_BYTES 41591552
+#define SG_STATIC_SCRATCH_BYTES 41599488
 enum sg_validation_flags {
     SG_VALID_MASS = 1,       /* a mass <= 0                                 */
     SG_VALID_RANGE2 = 2,     /* a range2 == 0                               */

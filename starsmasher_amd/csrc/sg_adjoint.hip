@@ -387,8 +387,9 @@ void k_adj_far(AdjArgs a)
         part[4 * a.n + i] = acc[8];
     }
     if (a.stats && lane == 0) {
-        atomicAdd(&a.stats[0], (unsigned long long)((t1 - t0 + tstep - 1) / tstep) * CHUNKS);
-        atomicAdd(&a.stats[1], (unsigned long long)nearv);
+        unsigned long long *st = stat_copy(a.stats);
+        atomicAdd(&st[0], (unsigned long long)((t1 - t0 + tstep - 1) / tstep) * CHUNKS);
+        atomicAdd(&st[1], (unsigned long long)nearv);
     }
 }
 

@@ -160,7 +160,7 @@ int ensure_capacity(sg_context *c, int64_t n, int64_t padded, int64_t ni)
         cap = 0;
         if ((rc = grow(&c->pb.err, &cap, 4)) != SG_OK) return rc;
         cap = 0;
-        if ((rc = grow(&c->stats, &cap, 4)) != SG_OK) return rc;
+        if ((rc = grow(&c->stats, &cap, sg::STAT_WORDS)) != SG_OK) return rc;
     }
     if (padded > c->cap_pad || !c->pb.sx) {
         int64_t cap = 0;
@@ -305,7 +305,7 @@ hipError_t enqueue_eval(const EvalPlan &p, const EvalBuffers &bufs, int n, int i
         a.psi = psi;
         a.psipart = bufs.psipart;
     }
-    SG_ENQ(launch_fill_zero_int((int *)bufs.stats, 8, s));
+    SG_ENQ(launch_fill_zero_int((int *)bufs.stats, 2 * STAT_WORDS, s));
 
     if (ev) SG_ENQ(hipEventRecord(ev[1], s));
     SG_ENQ(launch_pair_far(precision, a, p.wg, s));
@@ -527,7 +527,7 @@ int sg_psi_device(sg_context *c, int n, int ibeg, int ni, const double *x, const
     a.psi = psi;
     // the psi-only pass classifies no chunks: its statistics read zero
     // rather than the previous evaluation's counts
-    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 8, s));
+    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 2 * sg::STAT_WORDS, s));
     HIP_TRY(hipEventRecord(c->ev_k0, s));
     HIP_TRY(hipEventRecord(c->ev_kf, s));
     HIP_TRY(sg::launch_psi_only(a, s));
@@ -716,7 +716,7 @@ int sg_compute_batch_device(sg_context *c, int nsets, const long long *offsets, 
             c->gexec = nullptr;
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 8, s));
+            HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 2 * sg::STAT_WORDS, s));
             const hipError_t le = sg::launch_batch(precision, a, wg, ipl, nwg, s);
             const hipError_t ee = hipStreamEndCapture(s, &graph);
             HIP_TRY(le);
@@ -730,7 +730,7 @@ int sg_compute_batch_device(sg_context *c, int nsets, const long long *offsets, 
         }
         HIP_TRY(hipGraphLaunch(c->gexec, s));
     } else {
-        HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 8, s));
+        HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 2 * sg::STAT_WORDS, s));
         HIP_TRY(sg::launch_batch(precision, a, wg, ipl, nwg, s));
     }
     HIP_TRY(hipEventRecord(c->ev_kf, s));
@@ -806,10 +806,14 @@ int sg_last_precision_stats(sg_context *c, long long *visits, long long *near_vi
 {
     if (!c) return SG_ERR_ARG;
     if (!c->launched) return SG_ERR_STATE;
-    unsigned long long h[4] = {0, 0, 0, 0};
+    std::vector<unsigned long long> copies(sg::STAT_WORDS);
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(h, c->stats, sizeof(h), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(copies.data(), c->stats, sizeof(unsigned long long) * copies.size(),
+                      hipMemcpyDeviceToHost));
+    unsigned long long h[sg::STAT_N] = {0, 0, 0, 0};
+    for (int k = 0; k < sg::STAT_SLOTS; ++k)
+        for (int q = 0; q < sg::STAT_N; ++q) h[q] += copies[(size_t)k * sg::STAT_STRIDE + q];
     if (visits) *visits = (long long)h[0];
     if (near_visits) *near_visits = (long long)h[1];
     if (wide_visits) *wide_visits = (long long)h[2];
@@ -944,7 +948,7 @@ int sg_vjp_device(sg_context *c, int n, const double *x, const double *y, const 
     if ((rc = grow(&c->nearflag, &c->cap_flag, a.ngroups * ntiles)) != SG_OK) return rc;
     a.nearflag = c->nearflag;
     a.stats = c->stats;
-    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 8, s));
+    HIP_TRY(sg::launch_fill_zero_int((int *)c->stats, 2 * sg::STAT_WORDS, s));
 
     HIP_TRY(hipEventRecord(c->ev_k0, s));
     HIP_TRY(sg::launch_adj_far(precision, a, wg, s));

@@ -35,6 +35,12 @@ __device__ __forceinline__ float wave_max(float v)
     return uniform_f(v);
 }
 
+// This workgroup's copy of the statistics counters.
+__device__ __forceinline__ unsigned long long *stat_copy(unsigned long long *stats)
+{
+    return stats + (blockIdx.x % STAT_SLOTS) * STAT_STRIDE;
+}
+
 // Sum over the wave of a per-lane count, from ballots of its bits (no
 // cross-lane moves, so no lane-address registers stay live until the end of
 // a kernel for it).

@@ -532,11 +532,11 @@ __device__ __forceinline__ void far_block_fp32c(const SortedSet &I, const Sorted
         for (int k = 0; k < IPL; ++k) f += valid[k] ? nfp64[k] : 0u;
         const unsigned long long fw = wave_sum_count(f);
         if (lane == 0) {
-            atomicAdd(&stats[0],
-                      (unsigned long long)((b.t1 - b.t0 + b.tstep - 1) / b.tstep) * CHUNKS);
-            atomicAdd(&stats[1], (unsigned long long)nearv);
-            atomicAdd(&stats[2], (unsigned long long)widev);
-            atomicAdd(&stats[3], fw);
+            unsigned long long *st = stat_copy(stats);
+            atomicAdd(&st[0], (unsigned long long)((b.t1 - b.t0 + b.tstep - 1) / b.tstep) * CHUNKS);
+            atomicAdd(&st[1], (unsigned long long)nearv);
+            atomicAdd(&st[2], (unsigned long long)widev);
+            atomicAdd(&st[3], fw);
         }
     }
 }
@@ -639,8 +639,9 @@ __device__ __forceinline__ void far_block_fp64(const SortedSet &I, const SortedS
         b.part[3 * b.pstride + ii[k]] = ap[k];
     }
     if (stats && lane == 0) {
-        atomicAdd(&stats[0], (unsigned long long)((b.t1 - b.t0 + b.tstep - 1) / b.tstep) * CHUNKS);
-        atomicAdd(&stats[1], (unsigned long long)nearv);
+        unsigned long long *st = stat_copy(stats);
+        atomicAdd(&st[0], (unsigned long long)((b.t1 - b.t0 + b.tstep - 1) / b.tstep) * CHUNKS);
+        atomicAdd(&st[1], (unsigned long long)nearv);
     }
 }
 

@@ -24,6 +24,16 @@ struct TileMeta {
     double pad;
 };
 
+// Chunk-visit statistics (visits, near visits, wide visits, FP64 pairs):
+// STAT_SLOTS copies of the STAT_N counters, each copy on its own 128-byte
+// line; a workgroup adds to copy blockIdx.x % STAT_SLOTS and the host sums
+// the copies (one copy for all waves serialised their atomics in L2 and cost
+// up to a fifth of a small set's pass).
+constexpr int STAT_N = 4;
+constexpr int STAT_SLOTS = 64;
+constexpr int STAT_STRIDE = 16;                          // u64 per copy
+constexpr int STAT_WORDS = STAT_SLOTS * STAT_STRIDE;     // u64 in the buffer
+
 // Sorted, structure-of-arrays particle set as the pair kernels read it.
 //   x,y,z : FP64 positions
 //   m     : FP64 mass (0 for padding)
@@ -54,7 +64,7 @@ struct PairArgs {
     double *partial;              // [jsplit][4][ni] sorted-order partial sums
     unsigned char *nearflag;      // [ngroups][ntiles]: bit c set = chunk c of the tile may
                                   // hold softened pairs and takes the near kernel
-    unsigned long long *stats;    // [2]: chunk visits, near chunk visits (may be null)
+    unsigned long long *stats;    // [STAT_WORDS] statistics copies (may be null)
     double *psi;                  // grad-h softening term, input order (null: not formed)
     double *psipart;              // [jsplit][ni] its partial sums (with psi)
 };
@@ -105,7 +115,7 @@ struct AdjArgs {
     double *gx, *gy, *gz, *gm, *gH;     // outputs, input order
     double *partial;                    // [jsplit][5][n] sorted order
     unsigned char *nearflag;            // [ngroups][ntiles]
-    unsigned long long *stats;          // [2]
+    unsigned long long *stats;          // [STAT_WORDS] statistics copies
 };
 
 // Adjoint launchers (sg_adjoint.hip): far, near, reduce in stream order.
@@ -218,7 +228,7 @@ struct EvalBuffers {
     size_t sort_tmp_bytes;
     double *partial;
     unsigned char *nearflag;
-    unsigned long long *stats;   // [4]
+    unsigned long long *stats;   // [STAT_WORDS]
     double *psipart;             // with psi only
     bool merge_sort;             // sort with merge_sort_pairs (same order, no memset)
 };

@@ -62,7 +62,7 @@ constexpr Layout make_layout()
     L.red = take(sizeof(double) * 8 * NRED_BLOCKS);
     L.bbox = take(sizeof(double) * 8);
     L.err = take(sizeof(int) * 4);
-    L.stats = take(sizeof(unsigned long long) * 4);
+    L.stats = take(sizeof(unsigned long long) * sg::STAT_WORDS);
     for (int k = 0; k < 5; ++k) L.sj[k] = take(sizeof(double) * MAXPAD);
     L.sidx = take(sizeof(int) * MAXPAD);
     L.tiles = take(sizeof(sg::TileMeta) * (MAXPAD / TJ));

@@ -82,6 +82,21 @@ def test_static_matches_context_entry_point(prec):
 
 
 @pytest.mark.gpu
+def test_static_at_its_maximum_size():
+    """n = SG_STATIC_MAX_N (the largest partial-sum slab set and sort storage
+    the scratch holds) against the context entry point, FP32C, Wendland C4."""
+    L = _lib.load()
+    n = _lib.SG_STATIC_MAX_N
+    d = _set(n, 64)
+    got = _static(L, d, 0, n, 2, SG_PREC_FP32C)
+    with Gravity(0) as g:
+        want = g.compute(d["x"], d["y"], d["z"], d["m"], d["range2"], nkernel=2,
+                         precision=SG_PREC_FP32C)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
 def test_static_reports_validation_flags():
     """The device int given as err receives the validation bits: 0 for a
     valid set, SG_VALID_MASS after a set with a non-positive mass."""
