@@ -270,18 +270,24 @@ def err_summary(a, b):
 
 
 def time_config(g, d, desc, steps):
-    """One workload at the context's tile shapes: FP32C and FP64 steps
-    (inputs in HBM), FP32C against the FP64 kernel on every particle."""
+    """One workload at the library's default (automatic) tile shapes, in both
+    precisions: FP32C and FP64 steps (inputs in HBM), FP32C against the FP64
+    kernel on every particle."""
     from starsmasher_amd import SG_PREC_FP32C, SG_PREC_FP64
     from starsmasher_amd.hipmem import DeviceArray
     n = d["x"].size
+    saved = {p: g.get_config(p) for p in (SG_PREC_FP32C, SG_PREC_FP64)}
+    for p in saved:
+        g.set_config(p, 0, 1, 0)
     dev = [DeviceArray.from_host(d[k]) for k in ("x", "y", "z", "m", "range2")]
     r32 = time_precision(g, dev, n, SG_PREC_FP32C, steps, 2)
+    wg, ipl, js = g.last_config()
     r64 = time_precision(g, dev, n, SG_PREC_FP64, max(2, steps // 2), 1)
     for a in dev:
         a.free()
-    wg, ipl, _ = g.get_config(SG_PREC_FP32C)
-    out = {"workload": desc, "n": n,
+    for p, c in saved.items():
+        g.set_config(p, *c)
+    out = {"workload": desc, "n": n, "tile": {"wg": wg, "ipl": ipl, "jsplit": js},
            "fp32c": {"ms_per_step": round(r32["ms_per_step"], 4),
                      "kernel_ms": round(r32["far_ms"], 4),
                      "gpairs_per_s": round(float(n) ** 2 / (r32["ms_per_step"] * 1e-3) / 1e9, 3)},
@@ -568,10 +574,9 @@ def main():
                                                      for r in tr.table]}
         for o in outs:
             o.free()
-    wg, ipl, js = g.get_config(SG_PREC_FP32C)
-
     log(f"bench: FP32C n={n} steps={args.steps} warmup={args.warmup}")
     r32 = time_precision(g, dev, n, SG_PREC_FP32C, args.steps, args.warmup)
+    wg, ipl, js = g.last_config()   # the tile shape the timed steps ran with
     log(f"bench: FP32C {r32['ms_per_step']:.2f} ms/step (far {r32['far_ms']:.2f} ms)")
     r64 = None
     if not args.no_fp64:

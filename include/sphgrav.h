@@ -140,11 +140,16 @@ int sg_compute_host(sg_context *ctx, int n, int ibeg, int ni,
                     double *ax, double *ay, double *az, double *pot);
 
 /* Tile configuration of the pair kernel for a precision.
- * wg: threads per workgroup (64, 128 or 256); ipl: i-particles per lane
- * (1 or 2); jsplit: number of j-partitions per i-block (>= 1, partial sums
- * are reduced in FP64 in a fixed order); 0 for jsplit selects it from n. */
+ * wg: threads per workgroup (64, 128 or 256), or 0 (the default) to take 128
+ * for i-ranges of up to 65,536 particles and 256 above; ipl: i-particles per
+ * lane (1 or 2); jsplit: number of j-partitions per i-block (>= 1, partial
+ * sums are reduced in FP64 in a fixed order); 0 (the default) selects it
+ * from n. */
 int sg_set_config(sg_context *ctx, int precision, int wg, int ipl, int jsplit);
 int sg_get_config(sg_context *ctx, int precision, int *wg, int *ipl, int *jsplit);
+/* The tile shape the last evaluation on the context ran with (the automatic
+ * choices resolved); SG_ERR_STATE before the first one. */
+int sg_last_config(sg_context *ctx, int *wg, int *ipl, int *jsplit);
 
 /* Batch of independent particle sets (no reference counterpart: the
  * reference evaluates one set per call).  Set s holds particles
@@ -281,8 +286,9 @@ int sg_grav_psi(int n, int n_lower, int my_length,
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under stream capture, tight serving loops, harnesses that
  * forbid hipMalloc).  The evaluation of sg_compute_device -- same
- * arguments and results as a context at its default configuration (tile
- * shape 256 x 1, automatic j-split) -- on the CURRENT HIP device, enqueued
+ * arguments and results as a context at its default configuration
+ * (automatic workgroup size, one i per lane, automatic j-split) -- on the
+ * CURRENT HIP device, enqueued
  * on `stream` (a hipStream_t of that device; NULL = the null stream).  It
  * allocates nothing: every scratch buffer (sorted copies, tile boxes,
  * partial sums, sort storage, counters) is carved from one file-scope

@@ -42,7 +42,7 @@ EXPORTS = (
     "sg_set_config", "sg_get_config", "sg_last_timings", "sg_last_tile_stats", "sg_init_device",
     "sg_firsthalf", "sg_lasthalf", "sg_vjp_device", "sg_vjp_host", "sg_compute_psi_device",
     "sg_compute_psi_host", "sg_psi_device", "sg_psi_host", "sg_grav_psi", "sg_potential_energy",
-    "sg_last_precision_stats", "sg_compute_device_static",
+    "sg_last_precision_stats", "sg_compute_device_static", "sg_last_config",
 )
 
 
@@ -97,6 +97,7 @@ def load() -> C.CDLL:
     L.sg_last_kernel_ms.argtypes = [_vp, C.POINTER(C.c_float)]
     L.sg_set_config.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.sg_get_config.argtypes = [_vp, C.c_int, _ip, _ip, _ip]
+    L.sg_last_config.argtypes = [_vp, _ip, _ip, _ip]
     L.sg_last_timings.argtypes = [_vp] + [C.POINTER(C.c_float)] * 3
     L.sg_last_tile_stats.argtypes = [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.sg_last_precision_stats.argtypes = [_vp] + [C.POINTER(C.c_longlong)] * 4
@@ -121,7 +122,7 @@ def load() -> C.CDLL:
                  "sg_firsthalf", "sg_lasthalf", "sg_vjp_device", "sg_vjp_host",
                  "sg_compute_psi_device", "sg_compute_psi_host", "sg_psi_device",
                  "sg_psi_host", "sg_grav_psi", "sg_potential_energy", "sg_last_precision_stats",
-                 "sg_compute_device_static"):
+                 "sg_compute_device_static", "sg_last_config"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -46,7 +46,13 @@ def tune(g, n, inputs, outputs, precision=SG_PREC_FP32C, nkernel=0, candidates=N
     (x, y, z, m, range2); outputs: 4 device arrays of length n.
     """
     cands = candidates or CANDIDATES[precision]
-    incumbent = tuple(g.get_config(precision))
+    configured = tuple(g.get_config(precision))
+    incumbent = configured
+    if configured[0] == 0:
+        # the automatic workgroup size: the shape it resolves to at this n
+        g.compute_device(n, *inputs, *outputs, nkernel=nkernel, precision=precision)
+        g.synchronize()
+        incumbent = (g.last_config()[0],) + configured[1:]
     table = []
     for wg, ipl in cands:
         for js in jsplits:
@@ -68,7 +74,9 @@ def tune(g, n, inputs, outputs, precision=SG_PREC_FP32C, nkernel=0, candidates=N
                     f"{row[3]:.2f} ms (far {row[4]:.2f}, near {row[5]:.2f})")
     pick = _pick(table, incumbent, margin)
     best = tuple(pick[:3])
-    g.set_config(precision, *best)
+    # an automatic configuration that resolves to the pick stays automatic
+    # (it keeps choosing by size on other sets)
+    g.set_config(precision, *(configured if best == incumbent else best))
     return TuneResult(precision, best, pick[3], table)
 
 

@@ -204,7 +204,7 @@ hipError_t merge_sort_pairs(void *tmp, size_t tmp_bytes, const uint64_t *kin, ui
 
 // ------------------------------------------------- one evaluation (sg_capi.cpp)
 // Sizes of one evaluation of n particles with ni i-particles at a tile
-// shape; the context path (sg_compute_device) and the static-scratch path
+// shape (wg 0: chosen from ni); the context path (sg_compute_device) and the static-scratch path
 // (sg_compute_device_static) size their device buffers from it, so both
 // launch the same kernels with the same j-partitions.
 struct EvalPlan {
@@ -217,6 +217,9 @@ struct EvalPlan {
     int64_t psipart_elems; // grad-h partial sums: jsplit * ni (with psi; else 0)
 };
 int auto_jsplit(int niblocks, int wg, int ntiles, int64_t ni);
+// i-ranges up to this many particles take workgroups of 128 when the
+// configured workgroup size is 0 (automatic), larger ones 256
+constexpr int AUTO_SMALL_NI = 65536;
 // the HIP error that sg_status_string(SG_ERR_HIP) describes
 void record_hip_error(hipError_t e);
 EvalPlan plan_eval(int n, int ni, int wg, int ipl, int jsplit_cfg, int precision, bool psi);

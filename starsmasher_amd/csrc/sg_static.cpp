@@ -31,8 +31,10 @@ using sg::TJ;
 
 constexpr int64_t MAXN = SG_STATIC_MAX_N;
 constexpr int64_t MAXPAD = (MAXN + TJ - 1) / TJ * TJ;
-// Default tile shape of a context (sg_context in sg_capi.cpp): 256 x 1,
-// automatic j-split; at n <= MAXN the split is at most MAXPAD / TJ / 4.
+// Default tile shape of a context (sg_context in sg_capi.cpp): automatic
+// workgroup size (128 or 256, by the i-range), one i per lane, automatic
+// j-split; at n <= MAXN the split is at most MAXPAD / TJ / 4.  The near flags
+// are sized for 256, the larger count of the two sizes.
 constexpr int WG = 256, IPL = 1;
 constexpr int64_t MAX_JSPLIT = std::min<int64_t>(64, std::max<int64_t>(1, MAXPAD / TJ / 4));
 constexpr int NRED_BLOCKS = 1024;
@@ -140,7 +142,7 @@ extern "C" int sg_compute_device_static(int n, int ibeg, int ni, const double *x
     int rc = scratch_address(device, &base);
     if (rc != SG_OK) return rc;
 
-    const sg::EvalPlan p = sg::plan_eval(n, ni, WG, IPL, 0, precision, false);
+    const sg::EvalPlan p = sg::plan_eval(n, ni, 0, IPL, 0, precision, false);
     size_t sort_bytes = 0;
     hipError_t e = sg::merge_sort_pairs_temp_bytes(n, &sort_bytes);
     if (e != hipSuccess) {

@@ -216,9 +216,17 @@ class Gravity:
               "sg_set_config")
 
     def get_config(self, precision):
+        """(wg, ipl, jsplit) as configured; 0 = chosen per evaluation."""
         wg, ipl, js = C.c_int(), C.c_int(), C.c_int()
         check(self._L.sg_get_config(self._h, int(precision), C.byref(wg), C.byref(ipl), C.byref(js)),
               "sg_get_config")
+        return wg.value, ipl.value, js.value
+
+    def last_config(self):
+        """(wg, ipl, jsplit) the last evaluation ran with."""
+        wg, ipl, js = C.c_int(), C.c_int(), C.c_int()
+        check(self._L.sg_last_config(self._h, C.byref(wg), C.byref(ipl), C.byref(js)),
+              "sg_last_config")
         return wg.value, ipl.value, js.value
 
     # ----------------------------------------------------------- compute

@@ -241,6 +241,30 @@ def test_tile_configurations(grav, plummer16k, prec, wg, ipl, jsplit):
 
 
 @pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
+def test_automatic_workgroup_size(grav, prec):
+    """The default configuration (wg 0) takes 128-lane workgroups for an
+    i-range of up to 65,536 particles and 256 above, and gives the same bits
+    as asking for that shape explicitly."""
+    old = grav.get_config(prec)
+    assert old == (0, 1, 0)
+    for n, ibeg, ni, want_wg in ((20000, 0, 20000, 128), (70000, 0, 70000, 256),
+                                 (70000, 2000, 65536, 128)):
+        d = ics.plummer(n, seed=30 + want_wg)
+        auto = grav.compute(*_args(d), nkernel=1, precision=prec, ibeg=ibeg, ni=ni)
+        wg, ipl, js = grav.last_config()
+        print(f"[auto n={n} ni={ni}] resolved wg={wg} ipl={ipl} jsplit={js}")
+        assert (wg, ipl) == (want_wg, 1) and js >= 1
+        grav.set_config(prec, want_wg, 1, 0)
+        try:
+            expl = grav.compute(*_args(d), nkernel=1, precision=prec, ibeg=ibeg, ni=ni)
+            assert grav.last_config() == (wg, ipl, js)
+        finally:
+            grav.set_config(prec, *old)
+        for a, b in zip(auto, expl):
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
 def test_far_path_dominates_with_small_h(grav, prec):
     """Small softening lengths make almost every tile Newtonian, so the far
     kernel's FP32 body (not the FP64 near path) produces the result."""

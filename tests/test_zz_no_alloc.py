@@ -64,7 +64,7 @@ def test_static_matches_context_entry_point(prec):
     L = _lib.load()
     cases = [(_set(20000, 61), 0, 20000, 0), (_set(5000, 62), 1200, 3000, 2)]
     with Gravity(0) as g:
-        assert g.get_config(prec) == (256, 1, 0)
+        assert g.get_config(prec) == (0, 1, 0)   # the static entry point's own shape rule
         for k, (d, ibeg, ni, nk) in enumerate(cases):
             print(f"[static call {k}] n={d['x'].size} ibeg={ibeg} ni={ni} nk={nk} prec={prec}",
                   flush=True)
