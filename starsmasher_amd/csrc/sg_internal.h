@@ -192,7 +192,8 @@ hipError_t launch_gather_batch(const double *x, const double *y, const double *z
                                int64_t padded, double *ox, double *oy, double *oz, double *om,
                                double *oH, int *oidx, hipStream_t s);
 
-// Sorting (sg_sort.hip): radix sort of 63-bit keys with int payloads.
+// Sorting (sg_sort.hip): stable sort of 63-bit keys with int payloads (radix
+// sort; merge sort for sets of up to 65,536).
 hipError_t sort_pairs_temp_bytes(int n, size_t *bytes);
 hipError_t sort_pairs(void *tmp, size_t tmp_bytes, const uint64_t *kin, uint64_t *kout,
                       const int *vin, int *vout, int n, hipStream_t s);

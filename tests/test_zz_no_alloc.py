@@ -2,7 +2,8 @@
 scratch buffer in one file-scope device array (no device allocation).
 
 Its results are compared with sg_compute_device on a context at the default
-configuration (tile shape 256 x 1, automatic j-split), which runs the same
+configuration (automatic workgroup size, one i per lane, automatic j-split),
+which runs the same
 kernels with the same j-partitions and reduction order: the outputs must be
 bit-identical, in both precisions, on two calls in a row (the second with a
 smaller set and an i-subrange, so stale contents of the shared array from the
