@@ -77,7 +77,7 @@ struct sg_context {
     struct Plan {
         std::vector<int64_t> offsets;
         int wg = 0, ipl = 0;
-        int64_t padded = 0, ngroups = 0, nflags = 0;
+        int64_t padded = 0, ngroups = 0, nflags = 0, nparts = 0;
         int nitems = 0;
     } plan;
     int64_t *d_offsets = nullptr;
@@ -615,8 +615,15 @@ int sg_get_batch_config(sg_context *c, int precision, int *wg, int *ipl)
 
 namespace {
 
-// Host part of a batch plan: per-set layout, near-flag offsets and the work
-// items, longest sets first (their i-blocks cost the most tiles).
+// Host part of a batch plan: per-set layout, near-flag and partial-sum
+// offsets and the work items, longest first.  An item is one i-block of a
+// set against one j partition; sets of more than BATCH_JTILES tiles are cut
+// into ceil(ntiles / BATCH_JTILES) partitions, so that the longest items do
+// not outlast the rest of the queue (with whole sets, the 480-tile i-blocks
+// of the largest stars of a 64-star batch hold some workgroups 13% past the
+// mean finish of a greedy longest-first schedule; with partitions of at most
+// 64 tiles, 5%).  As in a single-set evaluation there are at most 64
+// partitions and at most 2^24 slot-partitions of partial sums (512 MB).
 int build_plan(sg_context *c, int nsets, const long long *offsets, int wg, int ipl)
 {
     auto &P = c->plan;
@@ -626,24 +633,35 @@ int build_plan(sg_context *c, int nsets, const long long *offsets, int wg, int i
     std::vector<sg::BatchSet> sets(nsets);
     std::vector<int2> items;
     const int64_t gi = (int64_t)wg * ipl;
-    int64_t pofs = 0, gofs = 0, fofs = 0;
+    int64_t slots = 0;
+    for (int s = 0; s < nsets; ++s) slots += (offsets[s + 1] - offsets[s] + TJ - 1) / TJ * TJ;
+    const int64_t jcap = std::max<int64_t>(1, std::min<int64_t>(64, ((int64_t)1 << 24) / slots));
+    int64_t pofs = 0, gofs = 0, fofs = 0, partofs = 0;
     for (int s = 0; s < nsets; ++s) {
         const int64_t n = offsets[s + 1] - offsets[s];
         const int64_t ntl = (n + TJ - 1) / TJ;
         const int64_t nblk = (n + gi - 1) / gi;
-        sets[s] = sg::BatchSet{pofs, n, (int)(pofs / TJ), (int)ntl, gofs, fofs};
+        const int js = (int)std::min<int64_t>(jcap, (ntl + sg::BATCH_JTILES - 1) / sg::BATCH_JTILES);
+        sets[s] = sg::BatchSet{pofs, n, (int)(pofs / TJ), (int)ntl, gofs, fofs, partofs, js};
         const int64_t groups = nblk * (wg / 64);   // every wave of every i-block
-        for (int64_t b = 0; b < nblk; ++b) items.push_back(make_int2(s, (int)b));
+        for (int64_t b = 0; b < nblk; ++b)
+            for (int p = 0; p < js; ++p) items.push_back(make_int2(s, (int)(b * js + p)));
         pofs += ntl * TJ;
         gofs += groups;
         fofs += groups * ntl;
+        partofs += (int64_t)js * 4 * ntl * TJ;   // >= 4 * pofs of the next set
     }
-    std::stable_sort(items.begin(), items.end(), [&](const int2 &u, const int2 &v) {
-        return sets[u.x].ntiles > sets[v.x].ntiles;
-    });
+    // tiles an item visits: partition p of a set holds ceil((ntiles - p) / js)
+    auto cost = [&](const int2 &u) {
+        const sg::BatchSet &st = sets[u.x];
+        return (st.ntiles - u.y % st.jsplit + st.jsplit - 1) / st.jsplit;
+    };
+    std::stable_sort(items.begin(), items.end(),
+                     [&](const int2 &u, const int2 &v) { return cost(u) > cost(v); });
     P.padded = pofs;
     P.ngroups = gofs;
     P.nflags = fofs;
+    P.nparts = partofs;
     P.nitems = (int)items.size();
     int rc;
     if ((rc = grow(&c->d_offsets, &c->cap_offsets, nsets + 1)) != SG_OK) return rc;
@@ -687,7 +705,7 @@ int sg_compute_batch_device(sg_context *c, int nsets, const long long *offsets, 
         if ((rc = build_plan(c, nsets, offsets, wg, ipl)) != SG_OK) return rc;
     }
     if ((rc = ensure_capacity(c, ntot, P.padded, 1)) != SG_OK) return rc;
-    if ((rc = grow(&c->partial, &c->cap_part, 4 * P.padded)) != SG_OK) return rc;
+    if ((rc = grow(&c->partial, &c->cap_part, P.nparts)) != SG_OK) return rc;
     if ((rc = grow(&c->nearflag, &c->cap_flag, std::max<int64_t>(P.nflags, 1))) != SG_OK) return rc;
 
     hipStream_t s = c->stream;

@@ -1012,17 +1012,21 @@ __device__ __forceinline__ Block batch_block(const BatchArgs &a, int item)
 {
     const int2 it = a.items[item];
     const BatchSet st = a.sets[it.x];
+    const int ib = it.y / st.jsplit, jp = it.y - ib * st.jsplit;
+    const int64_t pad = (int64_t)st.ntiles * TJ;
     Block b;
-    b.ibase = st.pofs + (int64_t)it.y * WG * IPL;
+    b.ibase = st.pofs + (int64_t)ib * WG * IPL;
     b.iend = st.pofs + st.n;
-    b.t0 = st.tile0;
+    b.t0 = st.tile0 + jp;
     b.t1 = st.tile0 + st.ntiles;
-    b.tstep = 1;
+    b.tstep = st.jsplit;
     b.fbase = st.tile0;
     b.fstride = st.ntiles;
-    b.flags = a.nearflag + st.flagofs + (int64_t)it.y * (WG / 64) * st.ntiles;
-    b.part = a.partial;
-    b.pstride = a.padded;
+    b.flags = a.nearflag + st.flagofs + (int64_t)ib * (WG / 64) * st.ntiles;
+    // partition jp's slab; slot i of the set sits at [i - pofs] (partofs >=
+    // pofs, so the base stays inside the buffer)
+    b.part = a.partial + (st.partofs - st.pofs) + (int64_t)jp * 4 * pad;
+    b.pstride = pad;
     return b;
 }
 
@@ -1075,23 +1079,43 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_near_batch(BatchArgs a)
     g.t0 = st.tile0;
     g.t1 = st.tile0 + st.ntiles;
     g.flags = a.nearflag + st.flagofs + lg * st.ntiles;
-    g.part = a.partial;
-    g.pstride = a.padded;
+    // added to the set's first partition slab (after the far kernel)
+    g.part = a.partial + (st.partofs - st.pofs);
+    g.pstride = (int64_t)st.ntiles * TJ;
     if (g.gbase >= g.iend || !group_has_near(g)) return;
     near_group<IPL, FAST, NK>(a.j, a.j, g, lds, s_red);
 }
 
-// Scatter the per-slot sums of a batch back to input order.
+// Sum the partition slabs of each slot in a fixed order (partition 0
+// first) and scatter the sums back to input order.
 __global__ __launch_bounds__(256) void k_reduce_batch(BatchArgs a)
 {
     const int64_t s = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (s >= a.padded) return;
     const int o = a.j.idx[s];
     if (o < 0) return;   // padding
-    a.ax[o] = a.partial[s];
-    a.ay[o] = a.partial[a.padded + s];
-    a.az[o] = a.partial[2 * a.padded + s];
-    a.pot[o] = a.partial[3 * a.padded + s];
+    // the set of slot s (sets are ordered by first slot)
+    int lo = 0, hi = a.nsets - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (a.sets[mid].pofs <= s) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchSet st = a.sets[lo];
+    const int64_t pad = (int64_t)st.ntiles * TJ;
+    const double *p = a.partial + st.partofs + (s - st.pofs);
+    double v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = p[c * pad];
+    for (int q = 1; q < st.jsplit; ++q) {
+        p += 4 * pad;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] += p[c * pad];
+    }
+    a.ax[o] = v[0];
+    a.ay[o] = v[1];
+    a.az[o] = v[2];
+    a.pot[o] = v[3];
 }
 
 inline dim3 grid_of(const PairArgs &a) { return dim3((unsigned)a.niblocks * (unsigned)a.jsplit); }

@@ -70,7 +70,11 @@ struct PairArgs {
 };
 
 // A batch of independent particle sets laid out back to back in one padded
-// sorted array; each set starts at a tile boundary.
+// sorted array; each set starts at a tile boundary.  A set of more than
+// BATCH_JTILES tiles has its j range cut into jsplit strided partitions
+// (partition p: tiles p, p + jsplit, ...), each with its own FP64 partial
+// sums, so that no work item holds a whole large set's j range.
+constexpr int BATCH_JTILES = 64;
 struct BatchSet {
     int64_t pofs;      // first padded slot of the set (multiple of TJ)
     int64_t n;         // real particles
@@ -78,6 +82,8 @@ struct BatchSet {
     int ntiles;        // tiles of the set
     int64_t gofs;      // first near group (groups of 64*ipl i)
     int64_t flagofs;   // first near flag
+    int64_t partofs;   // first partial sum: [jsplit][4][ntiles * TJ] from here
+    int jsplit;        // j partitions (>= 1)
 };
 
 struct BatchArgs {
@@ -86,12 +92,12 @@ struct BatchArgs {
     const TileMeta *chunks;
     const BatchSet *sets;
     int nsets;
-    const int2 *items;           // (set, i-block), longest first
+    const int2 *items;           // (set, i-block * jsplit + partition), longest first
     int nitems;
     int64_t ngroups;             // near groups over all sets
     int64_t padded;              // slots over all sets
     int nkernel;
-    double *partial;             // [4][padded]
+    double *partial;             // per set at partofs: [jsplit][4][ntiles * TJ]
     unsigned char *nearflag;
     unsigned *counter;           // work-queue head
     double *ax, *ay, *az, *pot;  // outputs, input order
