@@ -157,7 +157,9 @@ int sg_last_config(sg_context *ctx, int *wg, int *ipl, int *jsplit);
  * nsets+1 entries, offsets[0] = 0, every set non-empty); particles interact
  * only within their set.  Outputs (device, length offsets[nsets]) are in
  * input order.  All sets run in one persistent launch whose workgroups take
- * (set, i-block) items from a queue, longest sets first.  With use_graph the
+ * (set, i-block, j-partition) items from a queue, longest first (sets of
+ * more than 16,384 particles split their j range into partitions whose FP64
+ * partial sums are added in a fixed order).  With use_graph the
  * queue reset, far, near and scatter kernels are captured once in a HIP graph
  * and replayed while the plan, pointers and precision stay the same; the
  * preparation (validation, sort, gather) runs eagerly before the replay.
