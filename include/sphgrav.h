@@ -147,8 +147,10 @@ int sg_compute_host(sg_context *ctx, int n, int ibeg, int ni,
  * from n. */
 int sg_set_config(sg_context *ctx, int precision, int wg, int ipl, int jsplit);
 int sg_get_config(sg_context *ctx, int precision, int *wg, int *ipl, int *jsplit);
-/* The tile shape the last evaluation on the context ran with (the automatic
- * choices resolved); SG_ERR_STATE before the first one. */
+/* The tile shape the last single-set evaluation on the context ran with
+ * (sg_compute_device and the calls built on it, the automatic choices
+ * resolved); SG_ERR_STATE before one, or when the last call was of another
+ * kind (batch, psi-only, vector-Jacobian product). */
 int sg_last_config(sg_context *ctx, int *wg, int *ipl, int *jsplit);
 
 /* Batch of independent particle sets (no reference counterpart: the

@@ -465,6 +465,7 @@ int compute_device_impl(sg_context *c, int n, int ibeg, int ni, const double *x,
     if (ni > 0 && (!ax || !ay || !az || !pot)) return SG_ERR_ARG;
     HIP_TRY(hipSetDevice(c->device));
     c->launched = false;
+    c->last_shape[0] = 0;   // set again by a single-set pair pass
     if (ni == 0) return SG_OK;
 
     const sg::EvalPlan p = sg::plan_eval(n, ni, c->wg[precision], c->ipl[precision],
@@ -508,6 +509,7 @@ int sg_psi_device(sg_context *c, int n, int ibeg, int ni, const double *x, const
     if (ni > 0 && !psi) return SG_ERR_ARG;
     HIP_TRY(hipSetDevice(c->device));
     c->launched = false;
+    c->last_shape[0] = 0;   // set again by a single-set pair pass
     if (ni == 0) return SG_OK;
 
     const int64_t padded = ((int64_t)(n) + TJ - 1) / TJ * TJ;
@@ -694,6 +696,7 @@ int sg_compute_batch_device(sg_context *c, int nsets, const long long *offsets, 
     if (!x || !y || !z || !m || !range2 || !ax || !ay || !az || !pot) return SG_ERR_ARG;
     HIP_TRY(hipSetDevice(c->device));
     c->launched = false;
+    c->last_shape[0] = 0;   // set again by a single-set pair pass
     const int wg = c->bwg[precision], ipl = c->bipl[precision];
 
     auto &P = c->plan;
@@ -939,6 +942,7 @@ int sg_vjp_device(sg_context *c, int n, const double *x, const double *y, const 
     if (!gx || !gy || !gz || !gm || !gH) return SG_ERR_ARG;
     HIP_TRY(hipSetDevice(c->device));
     c->launched = false;
+    c->last_shape[0] = 0;   // set again by a single-set pair pass
 
     const int64_t padded = ((int64_t)(n) + TJ - 1) / TJ * TJ;
     int rc = ensure_capacity(c, n, padded, 1);

@@ -35,7 +35,8 @@
 //
 // Two launch forms share the block-level code: one particle set (grid over
 // i-blocks x j-partitions), and a batch of independent sets in one persistent
-// launch whose workgroups pull (set, i-block) items from a queue.
+// launch whose workgroups pull (set, i-block, j-partition) items from a
+// queue.
 #include "sg_internal.h"
 #include "sg_device.h"
 
@@ -1031,8 +1032,8 @@ __device__ __forceinline__ Block batch_block(const BatchArgs &a, int item)
 }
 
 // Persistent far kernel over a batch: every workgroup takes the next
-// (set, i-block) item from the queue until the queue is empty, so every
-// workgroup reaches the exit once all items are taken.
+// (set, i-block, j-partition) item from the queue until the queue is empty,
+// so every workgroup reaches the exit once all items are taken.
 template <int WG, int IPL, bool FP64, int NK>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(IPL == 1 && !FP64 ? 7 : 1)))
 void k_far_batch(BatchArgs a)

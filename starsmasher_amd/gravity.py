@@ -223,7 +223,9 @@ class Gravity:
         return wg.value, ipl.value, js.value
 
     def last_config(self):
-        """(wg, ipl, jsplit) the last evaluation ran with."""
+        """(wg, ipl, jsplit) the last single-set evaluation ran with; raises
+        SphgravError (SG_ERR_STATE) before one or after a call of another
+        kind (batch, psi-only, vector-Jacobian product)."""
         wg, ipl, js = C.c_int(), C.c_int(), C.c_int()
         check(self._L.sg_last_config(self._h, C.byref(wg), C.byref(ipl), C.byref(js)),
               "sg_last_config")

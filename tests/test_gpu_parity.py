@@ -264,6 +264,23 @@ def test_automatic_workgroup_size(grav, prec):
             np.testing.assert_array_equal(a, b)
 
 
+def test_last_config_follows_single_set_calls(grav):
+    """sg_last_config reports the shape of the last single-set pass; after a
+    batch or a psi-only call it reports SG_ERR_STATE instead of a stale one."""
+    d = ics.plummer(3000, seed=4)
+    grav.compute(*_args(d), nkernel=0)
+    assert grav.last_config()[:2] == (128, 1)
+    grav.compute_batch([d])
+    with pytest.raises(_lib.SphgravError) as e:
+        grav.last_config()
+    assert e.value.status == _lib.SG_ERR_STATE
+    grav.compute(*_args(d), nkernel=0)
+    assert grav.last_config()[:2] == (128, 1)
+    grav.psi(*_args(d), nkernel=0)
+    with pytest.raises(_lib.SphgravError):
+        grav.last_config()
+
+
 @pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
 def test_far_path_dominates_with_small_h(grav, prec):
     """Small softening lengths make almost every tile Newtonian, so the far
