@@ -141,9 +141,10 @@ def test_batch_configurations(grav, prec, wg, ipl):
 def test_batch_partitioned_sets(grav):
     """Sets of more than 64 tiles (16,384 particles) have their j range cut
     into strided partitions with partial sums of their own: 40,000 particles
-    (three partitions) and 20,000 (two) beside a one-partition set, both
-    precisions against oracle rows; the graph replay equals the eager batch."""
-    sets = _sets([40000, 3000, 20000], seed=11)
+    (three partitions), 20,000 and 16,385 (65 tiles: two) beside a
+    one-partition set, both precisions against oracle rows; the graph replay
+    equals the eager batch."""
+    sets = _sets([40000, 3000, 20000, 16385], seed=11)
     rng = np.random.default_rng(2)
     for prec in (SG_PREC_FP32C, SG_PREC_FP64):
         res = grav.compute_batch(sets, precision=prec)
