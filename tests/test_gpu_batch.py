@@ -138,25 +138,28 @@ def test_batch_configurations(grav, prec, wg, ipl):
                oracle.direct(d["x"], d["y"], d["z"], d["m"], d["range2"], nkernel=0))
 
 
-def test_batch_partitioned_sets(grav):
+@pytest.mark.parametrize("nk", [0, 1, 2])
+def test_batch_partitioned_sets(grav, nk):
     """Sets of more than 64 tiles (16,384 particles) have their j range cut
     into strided partitions with partial sums of their own: 40,000 particles
     (three partitions), 20,000 and 16,385 (65 tiles: two) beside a
-    one-partition set, both precisions against oracle rows; the graph replay
-    equals the eager batch."""
+    one-partition set, both precisions and the three softening kernels
+    against oracle rows; the graph replay equals the eager batch."""
     sets = _sets([40000, 3000, 20000, 16385], seed=11)
     rng = np.random.default_rng(2)
     for prec in (SG_PREC_FP32C, SG_PREC_FP64):
-        res = grav.compute_batch(sets, precision=prec)
-        again = grav.compute_batch(sets, precision=prec, use_graph=True)
+        res = grav.compute_batch(sets, nkernel=nk, precision=prec)
+        again = grav.compute_batch(sets, nkernel=nk, precision=prec, use_graph=True)
         for k, (d, r, r2) in enumerate(zip(sets, res, again)):
             for a, b in zip(r, r2):
                 np.testing.assert_array_equal(a, b)
             n = d["x"].size
             rows = np.unique(np.concatenate([rng.choice(n, min(n, 300), replace=False),
                                              [0, n - 1]]))
-            ref = oracle.direct(d["x"], d["y"], d["z"], d["m"], d["range2"], nkernel=0, rows=rows)
-            _check(f"partitioned set{k} n={n} prec={prec}", prec, [x[rows] for x in r], ref)
+            ref = oracle.direct(d["x"], d["y"], d["z"], d["m"], d["range2"], nkernel=nk,
+                                rows=rows)
+            _check(f"partitioned nk={nk} set{k} n={n} prec={prec}", prec, [x[rows] for x in r],
+                   ref)
 
 
 def test_batch_argument_errors(grav):
