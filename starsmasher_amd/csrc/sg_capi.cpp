@@ -369,9 +369,15 @@ int sg_create(int device, sg_context **out)
     HIP_TRY(hipSetDevice(device));
     sg_context *c = new sg_context();
     c->device = device;
+    // The events that mark the start and the phase boundaries of a pass only
+    // time it: they skip the system-scope fence (an L2 writeback and
+    // invalidate between the kernels of a pass).  The closing event keeps it.
+    const unsigned timing_only = hipEventDisableSystemFence;
     if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev_p0) != hipSuccess || hipEventCreate(&c->ev_k0) != hipSuccess ||
-        hipEventCreate(&c->ev_kf) != hipSuccess || hipEventCreate(&c->ev_k1) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_p0, timing_only) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_k0, timing_only) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_kf, timing_only) != hipSuccess ||
+        hipEventCreate(&c->ev_k1) != hipSuccess) {
         delete c;
         return SG_ERR_HIP;
     }
