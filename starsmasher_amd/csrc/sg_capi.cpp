@@ -627,11 +627,12 @@ namespace {
 // offsets and the work items, longest first.  An item is one i-block of a
 // set against one j partition; sets of more than BATCH_JTILES tiles are cut
 // into ceil(ntiles / BATCH_JTILES) partitions, so that the longest items do
-// not outlast the rest of the queue (with whole sets, the 480-tile i-blocks
-// of the largest stars of a 64-star batch hold some workgroups 13% past the
-// mean finish of a greedy longest-first schedule; with partitions of at most
-// 64 tiles, 5%).  As in a single-set evaluation there are at most 64
-// partitions and at most 2^24 slot-partitions of partial sums (512 MB).
+// not outlast the rest of the queue.  Where one large set dominates a batch
+// this is what keeps the launch short (a 262,144-particle star with 31 small
+// ones: 38.0 ms with whole-set items, 22.9 ms with partitions); on the
+// 64-star batch of N in [2,048, 131,072] it gains 0.5%.  As in a single-set
+// evaluation there are at most 64 partitions and at most 2^24
+// slot-partitions of partial sums (512 MB).
 int build_plan(sg_context *c, int nsets, const long long *offsets, int wg, int ipl)
 {
     auto &P = c->plan;
