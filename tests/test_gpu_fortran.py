@@ -126,6 +126,53 @@ def test_reference_gravforce_call_site(tmp_path, golden_dumps, nlo, nhi):
     assert ea.max() < 1e-5 and ep.max() < 1e-6
 
 
+def test_reference_gravforce_call_site_at_2p20(tmp_path):
+    """The reference's gravforce and the lasthalf call of uvdots (grav.f,
+    gpu_grav.f, compiled unmodified against starsmasher.h with nmax raised to
+    1,100,000, the one change a run at N = 2^20 needs) on the 2^20-particle
+    giant star with its central point mass, for the second half of the
+    particles as one rank's chunk.  The result is bit-identical to the
+    library's own evaluation of that i-range (grav.f forms range2 as
+    4 hp^2, negated for u = 0, which is what the set holds), and agrees with
+    the oracle on sampled rows and at the point mass.  grav.f's automatic
+    range(ntot) takes 8 MB of stack at this size: the run gets 64 MB."""
+    import resource
+    exe = _ref_exe("refsite_gravity_1m")
+    need = 64 << 20
+    _, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    if hard != resource.RLIM_INFINITY and hard < need:
+        pytest.skip(f"the stack hard limit ({hard} B) is below the {need} B grav.f needs at 2^20")
+    d = ics.giant_with_core(1 << 20)
+    n = d["x"].size
+    nlo, nhi = n // 2 + 1, n
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    _write_input(fin, d, nlo, nhi, 0)
+    env = dict(os.environ, SPHGRAV_QUIET="1")
+    env.pop("SPHGRAV_PRECISION", None)
+    print(f"[grav.f at 2^20] running rows {nlo}..{nhi}", flush=True)
+    r = subprocess.run(["bash", "-c", f'ulimit -s {need >> 10} && "$0" "$1" "$2"', exe,
+                        str(fin), str(fout)], capture_output=True, text=True, timeout=200,
+                       env=env)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    assert "1sthalf:" in r.stdout
+    out = _read_output(fout, n)
+    lo, hi = nlo - 1, nhi
+    assert np.all(out[:, :lo] == -7777.0)
+    from starsmasher_amd import Gravity
+    with Gravity(0) as g:
+        want = g.compute(d["x"], d["y"], d["z"], d["m"], d["range2"], ibeg=lo, ni=hi - lo)
+    for a, b in zip(out[:, lo:hi], want):
+        np.testing.assert_array_equal(a, b)
+    rng = np.random.default_rng(7)
+    rows = np.unique(np.concatenate([rng.choice(np.arange(lo, hi), 48, replace=False), [n - 1]]))
+    ref = oracle.direct(d["x"], d["y"], d["z"], d["m"], d["range2"], nkernel=0, rows=rows)
+    ea, ep = rel_errors(out[:3, rows].T, out[3, rows], np.stack(ref[:3], -1), ref[3])
+    print(f"[grav.f at 2^20] vs oracle on {rows.size} rows: acc max {ea.max():.3e} "
+          f"pot max {ep.max():.3e}")
+    assert ea.max() < 1e-5 and ep.max() < 1e-6
+
+
 def _collision_fixture():
     g = np.load(os.path.join(GOLDEN, "collision_run.npz"))
     return {k: g[k] for k in g.files}
