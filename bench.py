@@ -46,10 +46,12 @@ PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector (AMD spec)
 PEAK_FP64_TFLOPS = 78.6
 TC = 64                       # j per chunk (the unit of the near / wide decisions)
 
-# What the north star allows in FP64, and where this build adds FP64 beyond it.
+# The FP64 steps of the precision specification (FP32 pair arithmetic with
+# hi/lo positions and compensated sums, FP64 only for these), and the FP64
+# this library adds beyond them.
 PRECISION_POLICY = {
-    "north_star_fp64": ["origin subtraction", "cross-tile final reduction",
-                        "total-energy diagnostic"],
+    "specified_fp64": ["origin subtraction", "cross-tile final reduction",
+                       "total-energy diagnostic"],
     "added_fp64": [
         "wide chunks: pair differences formed in FP64 and rounded once (chunks close to a wave "
         "relative to its spread about the workgroup origin)",

@@ -20,7 +20,7 @@ def _have_gpu() -> bool:
         n = C.c_int(0)
         return _lib.load().sg_device_count(C.byref(n)) == 0 and n.value > 0
     except FileNotFoundError:
-        # on a GPU box a missing library is an error, not a skip
+        # on a machine with a GPU a missing library is an error, not a skip
         return os.path.exists("/dev/kfd")
 
 
