@@ -73,6 +73,8 @@ def test_argument_errors_are_status_codes():
     assert L.sg_firsthalf(4, 1, 4, *p, 0) == _lib.SG_ERR_NONFINITE
     assert L.sg_lasthalf(*[a.ctypes.data_as(dp) for a in _arrays(4)[:4]]) == _lib.SG_ERR_STATE
     assert L.sg_set_config(None, 0, 64, 2, 0) == _lib.SG_ERR_ARG
+    assert L.sg_set_config(None, 0, 0, 1, 0) == _lib.SG_ERR_ARG     # automatic size, no context
+    assert L.sg_last_config(None, None, None, None) == _lib.SG_ERR_ARG
     assert L.sg_compute_device(None, 4, 0, 4, *([None] * 5), 0, 0, *([None] * 4)) == _lib.SG_ERR_ARG
     e = C.c_double()
     assert L.sg_potential_energy(None, 4, None, None, C.byref(e)) == _lib.SG_ERR_ARG
