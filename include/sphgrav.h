@@ -302,9 +302,9 @@ int sg_grav_psi(int n, int n_lower, int my_length,
  *   - every call on a device shares that device's array and leaves its
  *     contents behind: calls that overlap on one device (different streams
  *     or host threads) race; calls in one stream are ordered;
- *   - the call issues kernels only (no memset or copy; the Morton sort is a
- *     merge sort with the same stable order as the context path's radix
- *     sort), so it may be captured in a graph, in global capture mode too;
+ *   - the call issues kernels only (no memset or copy; the Morton sort is
+ *     the merge sort the context path also takes at these sizes), so it may
+ *     be captured in a graph, in global capture mode too;
  *     the first call on a device must come before any capture (it looks up
  *     the array's device address);
  *   - input validation runs on the device: when `err` (a DEVICE int) is
