@@ -321,16 +321,23 @@ def test_waves_near_every_tile(grav, prec, wg, ipl, jsplit):
     _check(f"bigH {wg}x{ipl}/{jsplit}", prec, [r[rows] for r in res], ref)
 
 
+@pytest.fixture(scope="module")
+def giant64k():
+    return ics.giant_with_core(65536, seed=16)
+
+
 @pytest.mark.parametrize("prec", [SG_PREC_FP32C, SG_PREC_FP64])
-def test_giant_with_point_mass_rows(grav, prec):
+@pytest.mark.parametrize("nk", [0, 1, 2])
+def test_giant_with_point_mass_rows(grav, giant64k, prec, nk):
     """Masses over 4 decades, H over 3 decades and a 30% point mass (no self
-    potential, softening H = 0.2), against the oracle on sampled rows."""
-    d = ics.giant_with_core(65536, seed=16)
-    res = grav.compute(*_args(d), nkernel=0, precision=prec)
+    potential, softening H = 0.2), for the three softening kernels, against
+    the oracle on sampled rows."""
+    d = giant64k
+    res = grav.compute(*_args(d), nkernel=nk, precision=prec)
     rng = np.random.default_rng(1)
     rows = np.unique(np.concatenate([rng.choice(65536, 1500, replace=False), [65535]]))
-    ref = oracle.direct(*_args(d), nkernel=0, rows=rows)
-    _check("giant64k rows", prec, [r[rows] for r in res], ref)
+    ref = oracle.direct(*_args(d), nkernel=nk, rows=rows)
+    _check(f"giant64k rows nk={nk}", prec, [r[rows] for r in res], ref)
 
 
 @pytest.fixture(scope="module")
