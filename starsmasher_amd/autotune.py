@@ -3,10 +3,12 @@
 For a precision and a particle set already resident on the device, times
 every supported (workgroup size, i-particles per lane, j-partitions) shape of
 the far kernel and keeps the fastest whole evaluation (prep + far + near +
-reduce, HIP events).  The shape installed when tuning starts stays unless
+reduce, HIP events).  The shape installed when tuning starts (for the
+automatic workgroup size, the shape it resolves to on this set) stays unless
 another is faster by more than ``margin`` (default 2%), so that run-to-run
 noise does not flip between shapes of equal speed.  The chosen shape is
-installed on the context with ``sg_set_config`` and returned with the full
+installed on the context with ``sg_set_config`` (an automatic configuration
+that resolves to the pick is left automatic) and returned with the full
 timing table.
 """
 from __future__ import annotations
