@@ -569,8 +569,12 @@ def main():
     if args.tune:
         from starsmasher_amd.autotune import tune
         outs = [DeviceArray(n) for _ in range(4)]
+        # FP32C keeps its default shape unless another is 2% faster (the
+        # profiled shape); the FP64 comparison kernel takes its fastest
+        # measured shape, so the ratio is taken against its best configuration
         for prec in (SG_PREC_FP32C,) + (() if args.no_fp64 else (SG_PREC_FP64,)):
-            tr = tune(g, n, dev, outs, prec, reps=1, log=log)
+            tr = tune(g, n, dev, outs, prec, reps=1, log=log,
+                      margin=0.02 if prec == SG_PREC_FP32C else 0.0)
             tuning["fp32c" if prec == SG_PREC_FP32C else "fp64"] = {
                 "best": list(tr.best), "table_ms": [[r[0], r[1], r[2], round(r[3], 2)]
                                                      for r in tr.table]}
