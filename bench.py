@@ -358,10 +358,14 @@ def bench_batch_leg(g, steps, tune, log_fn):
     dev = [DeviceArray.from_host(np.concatenate([s[k] for s in stars]))
            for k in ("x", "y", "z", "m", "range2")]
     out = [DeviceArray(ntot) for _ in range(4)]
-    tuning = None
+    tuning = tuning64 = None
     if tune:
         tr = tune_batch(g, off, dev, out, SG_PREC_FP32C, reps=1, log=log_fn)
         tuning = {"best": list(tr.best), "table_ms": [[r[0], r[1], round(r[3], 2)] for r in tr.table]}
+        # the FP64 comparison batch at its fastest measured shape (no margin)
+        tr = tune_batch(g, off, dev, out, SG_PREC_FP64, reps=1, margin=0.0, log=log_fn)
+        tuning64 = {"best": list(tr.best),
+                    "table_ms": [[r[0], r[1], round(r[3], 2)] for r in tr.table]}
 
     def run_batch(prec, k, graph=True):
         g.compute_batch_device(off, *dev, *out, precision=prec, use_graph=graph)
@@ -403,6 +407,7 @@ def bench_batch_leg(g, steps, tune, log_fn):
         worst["pot_max"] = max(worst["pot_max"], e["pot_max"])
         worst["separate_acc_max"] = max(worst["separate_acc_max"], float(es.max()))
     wg, ipl = g.get_batch_config(SG_PREC_FP32C)
+    wg64, ipl64 = g.get_batch_config(SG_PREC_FP64)
     for a in dev + out:
         a.free()
     return {"workload": "64 stars, N log-uniform in [2048, 131072] (seed 4), Plummer / n=1.5 "
@@ -412,11 +417,12 @@ def bench_batch_leg(g, steps, tune, log_fn):
             "gpairs_per_s": round(pairs / (ms32 * 1e-3) / 1e9, 3),
             "separate_launches_ms": round(ms32_sep, 3),
             "batch_speedup_vs_separate": round(ms32_sep / ms32, 3),
-            "fp64_batch_ms": round(ms64, 3), "fp32c_over_fp64": round(ms64 / ms32, 3),
+            "fp64_batch_ms": round(ms64, 3), "fp64_tile": {"wg": wg64, "ipl": ipl64},
+            "fp32c_over_fp64": round(ms64 / ms32, 3),
             "per_star_worst_vs_fp64_batch": worst,
             "precision": precision_shares(st32, 1, 64 * ipl) | {
                 "fp64_pair_share": round(st32[3] / pairs, 8)},
-            "autotune": tuning}
+            "autotune": tuning, "autotune_fp64": tuning64}
 
 
 def time_adjoint(g, n_adj, steps, log_fn):
