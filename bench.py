@@ -272,10 +272,12 @@ def err_summary(a, b):
 
 
 def time_config(g, d, desc, steps):
-    """One workload at the library's default (automatic) tile shapes, in both
-    precisions: FP32C and FP64 steps (inputs in HBM), FP32C against the FP64
-    kernel on every particle."""
+    """One workload in both precisions: FP32C at the library's default
+    (automatic) tile shape, the FP64 comparison kernel at its fastest
+    measured shape; FP32C and FP64 steps (inputs in HBM), FP32C against the
+    FP64 kernel on every particle."""
     from starsmasher_amd import SG_PREC_FP32C, SG_PREC_FP64
+    from starsmasher_amd.autotune import tune
     from starsmasher_amd.hipmem import DeviceArray
     n = d["x"].size
     saved = {p: g.get_config(p) for p in (SG_PREC_FP32C, SG_PREC_FP64)}
@@ -284,7 +286,12 @@ def time_config(g, d, desc, steps):
     dev = [DeviceArray.from_host(d[k]) for k in ("x", "y", "z", "m", "range2")]
     r32 = time_precision(g, dev, n, SG_PREC_FP32C, steps, 2)
     wg, ipl, js = g.last_config()
+    outs = [DeviceArray(n) for _ in range(4)]
+    tune(g, n, dev, outs, SG_PREC_FP64, reps=1, margin=0.0)
+    for o in outs:
+        o.free()
     r64 = time_precision(g, dev, n, SG_PREC_FP64, max(2, steps // 2), 1)
+    tile64 = dict(zip(("wg", "ipl", "jsplit"), g.last_config()))
     for a in dev:
         a.free()
     for p, c in saved.items():
@@ -293,7 +300,7 @@ def time_config(g, d, desc, steps):
            "fp32c": {"ms_per_step": round(r32["ms_per_step"], 4),
                      "kernel_ms": round(r32["far_ms"], 4),
                      "gpairs_per_s": round(float(n) ** 2 / (r32["ms_per_step"] * 1e-3) / 1e9, 3)},
-           "fp64": {"ms_per_step": round(r64["ms_per_step"], 4),
+           "fp64": {"ms_per_step": round(r64["ms_per_step"], 4), "tile": tile64,
                     "gpairs_per_s": round(float(n) ** 2 / (r64["ms_per_step"] * 1e-3) / 1e9, 3)},
            "fp32c_over_fp64": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
            "fp32c_vs_fp64_kernel_all": err_summary(r32["result"], r64["result"]),
